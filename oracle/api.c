@@ -1,0 +1,164 @@
+/* CPU ORACLE (test infrastructure only) -- flat entry points for the Python tests
+ * (ctypes) and for the CPU-baseline leg of bench.py.  No algorithm lives here. */
+#define _POSIX_C_SOURCE 199309L
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "oracle.h"
+
+static OMat mat_of(const double *b)
+{
+    return (OMat){{b[0], b[1], b[2]}, {b[3], b[4], b[5]}, {b[6], b[7], b[8]}};
+}
+
+OMesh *oracle_mesh_create(const double *min, const double *max, const int *n, const int *periodic)
+{
+    return omesh_create(min, max, n, periodic);
+}
+
+void oracle_mesh_generate(OMesh *mesh) { omesh_generate(mesh); }
+void oracle_mesh_destroy(OMesh *mesh) { omesh_destroy(mesh); }
+int oracle_periodic_source(const OMesh *mesh, int ghost) { return omesh_periodic_source(mesh, ghost); }
+
+void oracle_mesh_to_dual(int ne, int nn, const long *eptr, const long *eind, int ncommon,
+                         long *xadj_out, long *adjncy_out, long cap)
+{
+    long *xadj;
+    long *adjncy;
+    o_mesh_to_dual(ne, nn, eptr, eind, ncommon, &xadj, &adjncy);
+    memcpy(xadj_out, xadj, (ne + 1) * sizeof(long));
+    long cnt = (xadj[ne] < cap) ? xadj[ne] : cap;
+    memcpy(adjncy_out, adjncy, cnt * sizeof(long));
+    free(xadj);
+    free(adjncy);
+}
+
+OEqns *oracle_eqns_create(const OMesh *mesh, int navierstokes)
+{
+    return oeqns_create(mesh, navierstokes);
+}
+
+void oracle_eqns_destroy(OEqns *e) { oeqns_destroy(e); }
+
+/* flux/limiter names as accepted by the reference (NULL disables) */
+void oracle_eqns_config(OEqns *e, int space_order, const char *flux, int viscous,
+                        const char *limiter, double coefficient, double gamma, double mu,
+                        double prandtl)
+{
+    e->space_order = space_order;
+    e->flux = flux ? o_flux_by_name(flux) : O_FLUX_NONE;
+    e->viscous = viscous;
+    e->property[0] = gamma;
+    e->property[1] = mu;
+    e->property[2] = prandtl;
+    oeqns_set_limiter(e, o_limiter_by_name(limiter), coefficient);
+}
+
+void oracle_eqns_set_bc(OEqns *e, const char *entity, const char *name, const double *ctx)
+{
+    const OMesh *m = e->mesh;
+    for (int i = 0; i < e->num_bc; i++) {
+        if (!strcmp(m->entities.name[m->entities.num_inner + i], entity)) {
+            oeqns_set_bc(e, i, o_bc_by_name(name), ctx);
+            return;
+        }
+    }
+    o_fatal("invalid entity (%s)", entity);
+}
+
+double *oracle_eqns_primitive(OEqns *e) { return &e->primitive[0][0]; }
+double *oracle_eqns_conserved(OEqns *e) { return &e->conserved[0][0]; }
+double *oracle_eqns_eps2(OEqns *e) { return e->eps2; }
+
+void oracle_boundary(const OEqns *e, double *p) { o_boundary(e, (double (*)[5])p); }
+
+void oracle_gradient(const OEqns *e, double *p, double *g)
+{
+    o_gradient(e, (double (*)[5])p, (OVec (*)[5])g);
+}
+
+void oracle_limiter(const OEqns *e, const double *p, double *g)
+{
+    o_limiter(e, (const double (*)[5])p, (OVec (*)[5])g);
+}
+
+void oracle_derivative(const OEqns *e, double *p, double *d)
+{
+    o_derivative(e, (double (*)[5])p, (double (*)[5])d);
+}
+
+double oracle_timestep(const OEqns *e, const double *p)
+{
+    return o_timestep(e, (const double (*)[5])p);
+}
+
+void oracle_residual(const OEqns *e, const double *d, double *res)
+{
+    o_residual(e, (const double (*)[5])d, res);
+}
+
+double oracle_euler(OEqns *e, double *time, double *res, double max_step, double courant)
+{
+    return o_euler_step(e, time, res, max_step, courant);
+}
+
+double oracle_lserk(OEqns *e, double *time, double *res, double max_step, double courant,
+                    int time_order, int num_stages)
+{
+    return o_lserk(e, time, res, max_step, courant, time_order, num_stages);
+}
+
+double oracle_implicit_euler(OEqns *e, double *time, double *res, double max_step,
+                             double courant, double newton_tol, double krylov_tol, int dim)
+{
+    return o_implicit_euler(e, time, res, max_step, courant, newton_tol, krylov_tol, dim);
+}
+
+void oracle_gmres(const OEqns *e, const double *u, const double *d, const double *r, double *inc,
+                  double step, double norm, double fd_scale, double krylov_tol, int dim)
+{
+    o_gmres(e, (const double (*)[5])u, (const double (*)[5])d, (const double (*)[5])r,
+            (double (*)[5])inc, step, norm, fd_scale, krylov_tol, dim);
+}
+
+void oracle_flux(const char *name, double *flux, const double *l, const double *r,
+                 const double *property, const double *basis)
+{
+    o_convective(o_flux_by_name(name), flux, l, r, property, mat_of(basis));
+}
+
+void oracle_viscous(double *flux, const double *p, const double *g, const double *property,
+                    const double *basis)
+{
+    o_viscous(flux, p, (const OVec *)g, property, mat_of(basis));
+}
+
+void oracle_bc(const char *name, double *outer, const double *inner, const double *property,
+               const double *basis, const double *ctx)
+{
+    o_bc(o_bc_by_name(name), outer, inner, property, mat_of(basis), ctx);
+}
+
+void oracle_riemann(const double *l, const double *r, double gamma, double loc, double *out)
+{
+    o_riemann(l, r, gamma, loc, out);
+}
+
+void oracle_prim2cons(double *u, const double *p, const double *property)
+{
+    o_prim2cons(u, p, property);
+}
+
+void oracle_cons2prim(double *p, const double *u, const double *property)
+{
+    o_cons2prim(p, u, property);
+}
+
+/* wall-clock helper for the CPU baseline (monotonic seconds) */
+double oracle_wtime(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + (1e-9 * (double)ts.tv_nsec);
+}

@@ -1,0 +1,177 @@
+"""Loaders for the CPU oracle (oracle/liboracle.so) and, when it was built from the reference
+tree, the reference build (oracle/_ref/libteal_ref.so).  Test infrastructure only."""
+import ctypes as C
+import os
+
+import numpy as np
+
+from teal_amd import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libteal_ref.so")
+
+D = C.c_double
+DP = C.POINTER(C.c_double)
+IP = C.POINTER(C.c_int)
+LP = C.POINTER(C.c_long)
+VP = C.c_void_p
+S = C.c_char_p
+
+
+def _sig(lib, name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+def _bind_common(lib, pre):
+    _sig(lib, pre + "eqns_config", None, VP, C.c_int, S, C.c_int, S, D, D, D, D)
+    _sig(lib, pre + "eqns_set_bc", None, VP, S, S, DP)
+    _sig(lib, pre + "eqns_primitive", DP, VP)
+    _sig(lib, pre + "eqns_conserved", DP, VP)
+    _sig(lib, pre + "eqns_eps2", DP, VP)
+    _sig(lib, pre + "boundary", None, VP, DP)
+    _sig(lib, pre + "gradient", None, VP, DP, DP)
+    _sig(lib, pre + "limiter", None, VP, DP, DP)
+    _sig(lib, pre + "derivative", None, VP, DP, DP)
+    _sig(lib, pre + "timestep", D, VP, DP)
+    _sig(lib, pre + "residual", None, VP, DP, DP)
+    _sig(lib, pre + "euler", D, VP, DP, DP, D, D)
+    _sig(lib, pre + "lserk", D, VP, DP, DP, D, D, C.c_int, C.c_int)
+    _sig(lib, pre + "implicit_euler", D, VP, DP, DP, D, D, D, D, C.c_int)
+    _sig(lib, pre + "gmres", None, VP, DP, DP, DP, DP, D, D, D, D, C.c_int)
+    _sig(lib, pre + "flux", None, S, DP, DP, DP, DP, DP)
+    _sig(lib, pre + "viscous", None, DP, DP, DP, DP, DP)
+    _sig(lib, pre + "bc", None, S, DP, DP, DP, DP, DP)
+    _sig(lib, pre + "riemann", None, DP, DP, D, D, DP)
+    _sig(lib, pre + "prim2cons", None, DP, DP, DP)
+    _sig(lib, pre + "cons2prim", None, DP, DP, DP)
+
+
+_oracle = None
+_ref = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+        lib = C.CDLL(ORACLE_SO)
+        _sig(lib, "oracle_mesh_create", VP, DP, DP, IP, IP)
+        _sig(lib, "oracle_mesh_generate", None, VP)
+        _sig(lib, "oracle_mesh_destroy", None, VP)
+        _sig(lib, "oracle_periodic_source", C.c_int, VP, C.c_int)
+        _sig(lib, "oracle_mesh_to_dual", None, C.c_int, C.c_int, LP, LP, C.c_int, LP, LP, C.c_long)
+        _sig(lib, "oracle_eqns_create", VP, VP, C.c_int)
+        _sig(lib, "oracle_eqns_destroy", None, VP)
+        _sig(lib, "oracle_wtime", D)
+        _bind_common(lib, "oracle_")
+        _oracle = lib
+    return _oracle
+
+
+def have_ref():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        lib = C.CDLL(REF_SO, mode=C.RTLD_GLOBAL)
+        _sig(lib, "ref_init", None)
+        _sig(lib, "ref_mesh_create", VP, DP, DP, IP, IP)
+        _sig(lib, "ref_mesh_validate", None, VP)
+        _sig(lib, "ref_eqns_create", VP, VP, C.c_int)
+        _bind_common(lib, "ref_")
+        lib.ref_init()
+        _ref = lib
+    return _ref
+
+
+def dp(a):
+    return a.ctypes.data_as(DP)
+
+
+def arr3(vals, typ=C.c_double):
+    return (typ * 3)(*vals)
+
+
+class Case:
+    """A mesh + equations configuration shared by the oracle, the reference build and the
+    device library.  `kind` is "euler" or "navierstokes"."""
+
+    def __init__(self, n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), periodic=(1, 1, 1),
+                 kind="euler", space_order=2, flux="hllc", limiter="minmod", coefficient=0.0,
+                 gamma=1.4, mu=0.0, prandtl=0.71, bcs=()):
+        self.n = tuple(n)
+        self.lo, self.hi, self.periodic = tuple(lo), tuple(hi), tuple(periodic)
+        self.kind = kind
+        self.space_order = space_order
+        self.flux = flux
+        self.limiter = limiter
+        self.coefficient = coefficient
+        self.gamma, self.mu, self.prandtl = gamma, mu, prandtl
+        self.bcs = list(bcs)  # (entity, name, ctx or None)
+
+    @property
+    def viscous(self):
+        return 1 if self.kind == "navierstokes" else 0
+
+    def config_args(self):
+        return (self.space_order, self.flux.encode() if self.flux else None, self.viscous,
+                self.limiter.encode() if self.limiter else None, self.coefficient,
+                self.gamma, self.mu, self.prandtl)
+
+
+def oracle_mesh(case):
+    lib = oracle()
+    m = lib.oracle_mesh_create(arr3(case.lo), arr3(case.hi), arr3(case.n, C.c_int),
+                               arr3(case.periodic, C.c_int))
+    lib.oracle_mesh_generate(m)
+    return m
+
+
+def oracle_eqns(case, mesh):
+    lib = oracle()
+    e = lib.oracle_eqns_create(mesh, 1 if case.kind == "navierstokes" else 0)
+    lib.oracle_eqns_config(e, *case.config_args())
+    for ent, name, ctx in case.bcs:
+        c = None if ctx is None else (C.c_double * 5)(*ctx)
+        lib.oracle_eqns_set_bc(e, ent.encode(), name.encode(), c)
+    return e
+
+
+def ref_eqns(case, mesh):
+    """Reference equations on the (layout-identical) oracle mesh."""
+    lib = ref()
+    e = lib.ref_eqns_create(mesh, 1 if case.kind == "navierstokes" else 0)
+    lib.ref_eqns_config(e, *case.config_args())
+    for ent, name, ctx in case.bcs:
+        c = None if ctx is None else (C.c_double * 5)(*ctx)
+        lib.ref_eqns_set_bc(e, ent.encode(), name.encode(), c)
+    return e
+
+
+def tgv_state(center, gamma=1.4, mach=0.1):
+    """Taylor-Green vortex primitive state {rho, u, v, w, p} at cell centres (rho = 1)."""
+    x, y, z = center[:, 0], center[:, 1], center[:, 2]
+    p0 = 1.0 / (gamma * mach * mach)
+    P = np.zeros((len(center), 5))
+    P[:, 0] = 1.0
+    P[:, 1] = np.sin(x) * np.cos(y) * np.cos(z)
+    P[:, 2] = -np.cos(x) * np.sin(y) * np.cos(z)
+    P[:, 3] = 0.0
+    P[:, 4] = p0 + (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2) / 16
+    return P
+
+
+def random_state(num, seed, rho=(0.5, 1.5), vel=0.5, p=(0.5, 1.5)):
+    rng = np.random.default_rng(seed)
+    P = np.zeros((num, 5))
+    P[:, 0] = rng.uniform(*rho, num)
+    P[:, 1:4] = rng.uniform(-vel, vel, (num, 3))
+    P[:, 4] = rng.uniform(*p, num)
+    return P
