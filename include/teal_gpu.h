@@ -1,0 +1,117 @@
+// teal_gpu.h -- C-ABI of the MI355X residual path (plain pointers and sizes, no torch types).
+//
+// These entry points sit behind the reference's residual interface: each one replaces the
+// CPU routine cited next to it and takes the same arrays in the reference's host layouts
+// (primitive num_cells x 5 AoS, derivative num_inner x 5, gradient num_cells x 5 x 3).
+// The teal API in equations.h / simulation.h is implemented on top of them; tests and
+// bench.py call them directly through ctypes.
+//
+// Errors: every entry point returns 0 (or a finite value) on success; on failure it returns
+// nonzero (or NaN) and teal_gpu_last_error() describes the failure.  Calls on one TealGpu
+// are serialized on its own HIP stream and are not thread-safe (like the reference).
+#pragma once
+
+#include "mesh.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct TealGpu TealGpu;
+
+enum TealPrecision {
+    TEAL_FP64 = 0,  // FP64 storage and arithmetic: bit-compatible with the reference
+    TEAL_MIXED = 1, // FP32 states/gradients/geometry/flux math, compensated FP32 face
+                    // accumulation, FP64 solution update, dt and residual norm
+};
+
+// flux ids follow the reference's name table (src/euler/convective.c:593-623)
+enum TealFlux {
+    TEAL_FLUX_NONE = 0,
+    TEAL_FLUX_GODUNOV,
+    TEAL_FLUX_ROE,
+    TEAL_FLUX_HLL,
+    TEAL_FLUX_HLLC,
+    TEAL_FLUX_HLLE,
+    TEAL_FLUX_AUSMD,
+    TEAL_FLUX_AUSMDV,
+    TEAL_FLUX_LXF,
+    TEAL_FLUX_CENTRAL,
+};
+
+// boundary ids follow src/euler/boundary.c:273-297 and src/navierstokes/boundary.c:20-26
+enum TealBoundary {
+    TEAL_BC_NONE = 0,
+    TEAL_BC_SYMMETRY,
+    TEAL_BC_SUPERSONIC_INFLOW,
+    TEAL_BC_SUPERSONIC_OUTFLOW,
+    TEAL_BC_SUBSONIC_INFLOW,
+    TEAL_BC_SUBSONIC_OUTFLOW,
+    TEAL_BC_PRESSURE_OUTFLOW,
+    TEAL_BC_FARFIELD,
+    TEAL_BC_ADIABATIC_WALL,
+};
+
+typedef struct {
+    int space_order;           // 1 or 2 (src/equations.h:134)
+    int flux;                  // TealFlux (src/equations.h:137)
+    int viscous;               // 1: central viscous flux (src/navierstokes/viscous.c:11-39)
+    int limiter;               // 0 none, 1 Barth-Jespersen, 2 Venkatakrishnan (limiter.h:5-9)
+    int navierstokes;          // 1: navierstokes_timestep, 0: euler_timestep
+    double gamma, mu, prandtl; // properties (src/euler.h:19-21, src/navierstokes.h:19-23)
+    int num_boundary;          // entities.off_boundary - entities.num_inner
+    const int *bc;             // [num_boundary] TealBoundary per boundary entity
+    const double *bc_context;  // [num_boundary][5] primitive context state per entity
+    const double *eps2;        // [num_inner] Venkatakrishnan parameter (limiter.c:8-21), or
+                               // NULL: computed here as (coefficient cbrt(V))^3
+    double coefficient;        // Venkatakrishnan K (used when eps2 is NULL)
+} TealGpuConfig;
+
+// Upload a generated mesh (src/mesh.h:13-72 layout) into the device layout.
+// Replaces nothing in the reference (it has no device); called once per Equations.
+TealGpu *teal_gpu_create(const Mesh *mesh, int precision);
+void teal_gpu_destroy(TealGpu *gpu);
+int teal_gpu_configure(TealGpu *gpu, const TealGpuConfig *config);
+const char *teal_gpu_last_error(void);
+
+// Host-pointer residual path.  Replaces, one for one:
+int teal_gpu_boundary(TealGpu *gpu, double *primitive);          // src/equations/boundary.c:5
+int teal_gpu_gradient(TealGpu *gpu, double *primitive, double *gradient); // gradient.c:7
+int teal_gpu_limiter(TealGpu *gpu, const double *primitive, double *gradient); // limiter.c:115
+int teal_gpu_derivative(TealGpu *gpu, double *primitive, double *derivative); // derivative.c:269
+double teal_gpu_timestep(TealGpu *gpu, const double *primitive); // src/equations/timestep.c:7
+int teal_gpu_residual(TealGpu *gpu, const double *derivative, double *residual); // residual.c:8
+
+// Resident state: the device copy of the primitive variables is authoritative between
+// teal_gpu_upload and teal_gpu_download.
+int teal_gpu_upload(TealGpu *gpu, const double *primitive);
+int teal_gpu_download(TealGpu *gpu, double *primitive);
+
+// One step of the reference's Advance methods on the resident state; returns step0
+// (src/simulation/advance.c:11 euler, :47 lserk, :119 implicit_euler).
+double teal_gpu_euler(TealGpu *gpu, double *time, double *residual, double max_step,
+                      double courant);
+double teal_gpu_lserk(TealGpu *gpu, double *time, double *residual, double max_step,
+                      double courant, int time_order, int num_stages);
+double teal_gpu_implicit_euler(TealGpu *gpu, double *time, double *residual, double max_step,
+                               double courant, double newton_tolerance,
+                               double krylov_tolerance, int krylov_dimension);
+
+// `steps` lserk steps with no host synchronization between them (the inner loop of
+// src/simulation/run.c:68-102 without output); the final time is returned in *time.
+int teal_gpu_run_lserk(TealGpu *gpu, int steps, double *time, double courant, int time_order,
+                       int num_stages);
+
+// Per-kernel timing of the dominant (face flux) kernel with HIP events recorded on the
+// library's stream: enable, run, then read the mean duration (ms) and launch count.
+int teal_gpu_timing(TealGpu *gpu, int capacity);
+double teal_gpu_timing_flux_ms(TealGpu *gpu, int *launches);
+double teal_gpu_timing_grad_ms(TealGpu *gpu, int *launches);
+
+// Device-layout facts for measurement: counts and the algorithmic bytes moved per launch
+// of the gradient/limiter kernel and of the flux kernel (see the byte model in the docs).
+int teal_gpu_info(TealGpu *gpu, long *counts /* [6]: ni, nc, nf, nfi, nfb, W */);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,37 @@
+#!/bin/bash
+# Build libteal_amd.so for gfx950 in place (teal_amd/libteal_amd.so).
+#   FP64 kernels: -ffp-contract=off (bit-compatible with the reference)
+#   FP32 kernels: contraction on
+# Packed FP32 instructions are disabled (-packed-fp32-ops) on every HIP line.
+set -euo pipefail
+HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
+ROOT="$(cd "$HERE/../.." && pwd)"
+OUT="$HERE/../libteal_amd.so"
+OBJ="$HERE/../_build"
+mkdir -p "$OBJ"
+ARCH="${TEAL_ARCH:-gfx950}"
+JOBS="${TEAL_JOBS:-8}"
+HIPFLAGS=(-O3 -std=c++17 -fPIC --offload-arch="$ARCH" -I"$ROOT/include" -I"$HERE"
+          -Xclang -target-feature -Xclang -packed-fp32-ops -Wno-unused-result)
+CFLAGS=(-std=c11 -O2 -fPIC -ffp-contract=off -Wall -Wextra -Wno-unused-parameter -I"$ROOT/include")
+
+pids=()
+build_hip() {  # src flags...
+    local src="$1"; shift
+    /opt/rocm/bin/hipcc "${HIPFLAGS[@]}" "$@" -c "$HERE/$src" -o "$OBJ/${src%.*}.o"
+}
+build_hip device.hip -ffp-contract=off & pids+=($!)
+build_hip mixed.hip -ffp-contract=fast & pids+=($!)
+build_hip krylov.hip -ffp-contract=off & pids+=($!)
+build_hip plugins.hip -ffp-contract=off & pids+=($!)
+for src in core.c mesh_create.c mesh_generate.c mesh_util.c equations.c simulation.c; do
+    if [ -f "$HERE/$src" ]; then
+        gcc "${CFLAGS[@]}" -c "$HERE/$src" -o "$OBJ/${src%.*}.o" & pids+=($!)
+    fi
+done
+rc=0
+for p in "${pids[@]}"; do wait "$p" || rc=1; done
+[ $rc -eq 0 ] || { echo "build failed" >&2; exit 1; }
+/opt/rocm/bin/hipcc -shared -fPIC --offload-arch="$ARCH" -o "$OUT.tmp" "$OBJ"/*.o -lm
+mv "$OUT.tmp" "$OUT"
+echo "built $OUT"

@@ -1,0 +1,18 @@
+// plugins.h -- device ids of the host plugin callbacks (internal to the library).
+#pragma once
+
+#include "equations.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int teal_flux_id(Convective *fn);       // TealFlux id, -1 if the callback is not ours
+int teal_viscous_id(Viscous *fn);       // 1 for the central viscous flux, 0 none, -1 foreign
+int teal_bc_id(Boundary *fn);           // TealBoundary id, -1 if foreign
+int teal_timestep_id(Timestep *fn);     // 0 Euler, 1 Navier-Stokes, -1 foreign
+int teal_convert_is_euler(Convert *prim2cons, Convert *cons2prim);
+
+#ifdef __cplusplus
+}
+#endif
