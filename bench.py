@@ -1,0 +1,299 @@
+#!/usr/bin/env python3
+"""Benchmark: RK cell-updates/s of the Navier-Stokes Taylor-Green vortex on a periodic 128^3
+hex mesh (2,097,152 cells), FP64 GPU path vs mixed FP32/FP64 GPU path, with the mixed path's
+error against FP64, the flux kernel's HBM roofline fraction, and the reference CPU path timed
+on this host.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128]
+
+One step = one 3-stage LSERK step (src/simulation/advance.c:47-117) of every cell; the state
+stays resident in HBM for the whole timed region (no host transfers inside it).
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import teal_amd  # noqa: E402
+from teal_amd import _abi  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+GAMMA, MACH, RE, PR, CFL = 1.4, 0.1, 1600.0, 0.71, 0.5
+
+
+def tgv_primitive(center):
+    """Taylor-Green vortex, rho = 1, u = sin x cos y cos z, v = -cos x sin y cos z, w = 0,
+    p = 1/(gamma M^2) + (cos 2x + cos 2y)(cos 2z + 2)/16."""
+    x, y, z = center[:, 0], center[:, 1], center[:, 2]
+    P = np.zeros((len(center), 5))
+    P[:, 0] = 1.0
+    P[:, 1] = np.sin(x) * np.cos(y) * np.cos(z)
+    P[:, 2] = -np.cos(x) * np.sin(y) * np.cos(z)
+    P[:, 4] = 1.0 / (GAMMA * MACH * MACH) + (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2) / 16
+    return P
+
+
+def byte_model(ni, nc, nf, mixed, viscous=True, vk=False):
+    """Algorithmic HBM bytes: every operand array read once, every result written once.
+    s = state/geometry bytes (8 FP64, 4 mixed), u = 8 (FP64 U0/U), i = 4 (index).
+    `per_cell_update` is the path-level model of one RK step (3 residual passes + the
+    per-step dt/U0 work) per cell; the per-kernel figures are per launch of this library's
+    kernels: gradient+limiter; the stage kernel (face fluxes in LDS, accumulation, RK update);
+    and, for the split path, face fluxes (writes F) and accumulation (reads F)."""
+    s, u, i = (4 if mixed else 8), 8, 4
+    v = 1 if viscous else 0
+    k = 1 if vk else 0
+    m = 1 if mixed else 0
+    nn = 2 * nf - (nc - ni)  # incidences (CSR neighbour entries) of inner cells
+    nfv = 10 if viscous else 5  # values per face flux record
+    b_grad = 5 * s * nc + (2 * i + 6 * s) * nf + 15 * s * ni
+    b_lim = 5 * s * nc + 15 * s * ni + i * (ni + 1 + nn) + 3 * s * nn + k * s * ni + 5 * s * ni
+    b_flux = (25 * s * nc + (2 * i + 16 * s + 4 * v * s) * nf + s * ni + 5 * u * ni + 5 * s * ni
+              + m * 5 * u * ni)
+    b_step = 5 * (m * u + (1 - m) * s) * ni + 4 * s * ni + 5 * u * ni
+    faces = (2 * i + 16 * s + 4 * v * s) * nf  # face index pair + area, basis, offsets, correction
+    b_stage = 5 * s * nc + 20 * s * ni + faces + i * nf + i * nn + 5 * u * ni + s * ni + 5 * s * ni
+    b_face = faces + 5 * s * nc + 20 * s * ni + nfv * s * nf
+    b_accum = i * nn + nfv * s * nf + 5 * u * ni + s * ni + 5 * s * ni
+    return {"grad_limit": b_grad + b_lim, "flux": b_flux, "step": b_step, "stage": b_stage,
+            "face": b_face, "accum": b_accum,
+            "per_cell_update": (b_step + 3 * (b_grad + b_lim + b_flux)) / ni}
+
+
+def make_gpu(mesh, precision):
+    g = teal_amd.Gpu(mesh, precision)
+    g.configure(space_order=2, flux="hllc", viscous=1, limiter="minmod", navierstokes=1,
+                gamma=GAMMA, mu=1.0 / RE, prandtl=PR)
+    return g
+
+
+def timed_run(g, P0, steps, warmup, time_kernels):
+    lib = g.lib
+    t = C.c_double(0.0)
+    g.upload(P0)
+    if warmup:
+        teal_amd.check(lib.teal_gpu_run_lserk(g.h, warmup, C.byref(t), CFL, 2, 3), "warmup")
+    g.upload(P0)
+    if time_kernels:
+        teal_amd.check(lib.teal_gpu_timing(g.h, 3 * steps + 3), "timing")
+    t = C.c_double(0.0)
+    t0 = time.perf_counter()
+    rc = lib.teal_gpu_run_lserk(g.h, steps, C.byref(t), CFL, 2, 3)  # ends with a sync
+    t1 = time.perf_counter()
+    teal_amd.check(rc, "run_lserk")
+    out = {"seconds": t1 - t0, "sim_time": t.value}
+    if time_kernels:
+        nl = C.c_int(0)
+        for k, name in enumerate(("grad", "face", "stage")):
+            out[name + "_ms"] = lib.teal_gpu_timing_ms(g.h, k, C.byref(nl))
+            out[name + "_launches"] = nl.value
+        out["fused"] = out["face_launches"] == 0
+        lib.teal_gpu_timing(g.h, 0)
+    return out
+
+
+def conserved(P):
+    U = np.empty_like(P)
+    U[:, 0] = P[:, 0]
+    U[:, 1:4] = P[:, :1] * P[:, 1:4]
+    U[:, 4] = P[:, 4] / (GAMMA - 1) + 0.5 * P[:, 0] * np.sum(P[:, 1:4] ** 2, axis=1)
+    return U
+
+
+def accuracy(g64, gmx, P0, steps, volume):
+    """Advance both paths with the FP64 path's step sizes; L2/Linf of the conserved
+    variables of the mixed path against FP64 after `steps` steps."""
+    g64.upload(P0)
+    gmx.upload(P0)
+    t64 = tmx = 0.0
+    for _ in range(steps):
+        s0, t_new, _ = g64.lserk(t64, CFL)
+        dt = t_new - t64
+        t64 = t_new
+        _, tmx, _ = gmx.lserk(tmx, CFL * 1.05, max_step=dt)
+    ni = len(volume)
+    U64 = conserved(g64.download()[:ni])
+    Umx = conserved(gmx.download()[:ni])
+    d = Umx - U64
+    w = volume / volume.sum()
+    l2 = np.sqrt((w[:, None] * d * d).sum(axis=0))
+    scale = np.sqrt((w[:, None] * U64 * U64).sum(axis=0))
+    # momentum components relative to the momentum magnitude (w starts at zero in this flow)
+    scale[1:4] = np.sqrt((w * (U64[:, 1:4] ** 2).sum(axis=1)).sum())
+    names = ["rho", "rho_u", "rho_v", "rho_w", "E"]
+    return {"steps": steps, "time_fp64": t64, "time_mixed": tmx,
+            "l2": dict(zip(names, l2.tolist())),
+            "l2_relative": dict(zip(names, (l2 / scale).tolist())),
+            "linf": dict(zip(names, np.abs(d).max(axis=0).tolist())),
+            "l2_max": float(l2.max()), "l2_relative_max": float((l2 / scale).max())}
+
+
+def cpu_baseline(n, budget_s=20.0):
+    """The reference's own lserk (oracle/_ref, built from the reference sources) -- or, if it
+    is absent, the oracle restatement -- on the same TGV mesh, one core, a bounded number
+    of steps."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+
+    case = ol.Case((n, n, n), lo=(0, 0, 0), hi=(2 * np.pi,) * 3, kind="navierstokes", mu=1 / RE,
+                   prandtl=PR)
+    om = ol.oracle_mesh(case)
+    arrays = _abi.mesh_arrays(om)
+    nc, ni = arrays["num_cells"], arrays["num_inner"]
+    P0 = tgv_primitive(arrays["center"][:nc])
+    P0[ni:] = 0
+    kind = "reference"
+    try:
+        lib = ol.ref()
+        e = ol.ref_eqns(case, om)
+        prim = lib.ref_eqns_primitive
+        step = lambda t, r: lib.ref_lserk(e, C.byref(t), r, 1e300, CFL, 2, 3)
+    except OSError:
+        kind = "port"
+        lib = ol.oracle()
+        e = ol.oracle_eqns(case, om)
+        prim = lib.oracle_eqns_primitive
+        step = lambda t, r: lib.oracle_lserk(e, C.byref(t), r, 1e300, CFL, 2, 3)
+    p = np.ctypeslib.as_array(prim(e), shape=(nc * 5,))
+    p[:] = P0.ravel()
+    t = C.c_double(0.0)
+    r = (C.c_double * 5)()
+    done = 0
+    t0 = time.perf_counter()
+    while True:
+        step(t, r)
+        done += 1
+        if time.perf_counter() - t0 >= budget_s or done >= 50:
+            break
+    secs = time.perf_counter() - t0
+    return {"value": ni * done / secs, "unit": "cell-updates/s", "cores": 1, "kind": kind,
+            "sample": f"{done} LSERK step(s) of NS-TGV {n}^3 ({ni} cells), single thread, "
+                      f"{secs:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--accuracy-steps", type=int, default=None)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE/"
+                         "WRITE_SIZE against a kernel of known traffic)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
+              "(see the sharding design in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    if args.copy_probe:
+        import torch
+
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 1 << 30, "copy_probe_bytes_written": 1 << 30,
+                          "launches": 3}))
+        return
+
+    n = args.n
+    mesh = teal_amd.mesh_box((n, n, n), (0.0, 0.0, 0.0), (2 * np.pi,) * 3, (1, 1, 1))
+    a = _abi.mesh_arrays(mesh)
+    nc, ni, nf = a["num_cells"], a["num_inner"], a["num_faces"]
+    P0 = tgv_primitive(a["center"][:nc])
+    P0[ni:] = 0
+    volume = a["volume"][:ni].copy()
+
+    results = {}
+    gpus = {}
+    for name, prec in (("fp64", teal_amd.FP64), ("mixed", teal_amd.MIXED)):
+        g = make_gpu(mesh, prec)
+        gpus[name] = g
+        r = timed_run(g, P0, args.steps, args.warmup, time_kernels=True)
+        r["ms_per_step"] = 1e3 * r["seconds"] / args.steps
+        r["cell_updates_per_s"] = ni * args.steps / r["seconds"]
+        model = byte_model(ni, nc, nf, mixed=(prec == teal_amd.MIXED))
+        r["bytes_per_cell_update"] = model["per_cell_update"]
+        r["hbm_fraction_step"] = r["cell_updates_per_s"] * model["per_cell_update"] / (HBM_PEAK_GBS * 1e9)
+        kernels = (("grad", "grad_limit"), ("stage", "stage" if r["fused"] else "accum"))
+        if not r["fused"]:
+            kernels += (("face", "face"),)
+        for kern, key in kernels:
+            r[kern + "_bytes"] = model[key]
+            r[kern + "_gbs"] = model[key] / (r[kern + "_ms"] * 1e-3) / 1e9
+        results[name] = r
+        print(f"# {name}: {r['ms_per_step']:.3f} ms/step, {r['cell_updates_per_s']:.4g} "
+              f"cell-updates/s; " + ", ".join(f"{k} {r[k + '_ms']:.3f} ms ({r[k + '_gbs']:.0f} GB/s)"
+                                              for k, _ in kernels),
+              file=sys.stderr, flush=True)
+
+    acc_steps = args.accuracy_steps if args.accuracy_steps is not None else min(args.steps, 1000)
+    err = accuracy(gpus["fp64"], gpus["mixed"], P0, acc_steps, volume)
+    print(f"# mixed vs fp64 after {acc_steps} steps: L2 {err['l2_max']:.3g} "
+          f"(relative {err['l2_relative_max']:.3g})", file=sys.stderr, flush=True)
+    for g in gpus.values():
+        g.close()
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(n)
+        except Exception as exc:  # reported, not fatal
+            cpu = {"value": None, "error": str(exc)}
+
+    mx, fp = results["mixed"], results["fp64"]
+    KEYS = ["cell_updates_per_s", "ms_per_step", "hbm_fraction_step", "bytes_per_cell_update",
+            "fused", "grad_ms", "grad_gbs", "stage_ms", "stage_gbs", "face_ms", "face_gbs"]
+    timed = [k for k in ("grad", "face", "stage") if k + "_gbs" in mx]
+    dom = max(timed, key=lambda k: mx[k + "_ms"])  # dominant kernel
+    line = {
+        "metric": "RK cell-updates/s, TGV 128^3, mixed vs FP64 GPU at stated L2 error; %HBM",
+        "value": mx["cell_updates_per_s"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mx["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 flux/gradient/geometry + f64 update (mixed)",
+        "data": "synthetic",
+        "config": {
+            "workload": f"Navier-Stokes Taylor-Green vortex, periodic {n}^3 hex mesh on [0,2pi]^3 "
+                        f"({ni} cells), rho=1, M=0.1, Re=1600, Pr=0.71, HLLC + central viscous, "
+                        "LSQ gradients + Barth-Jespersen, LSERK order 2 / 3 stages, CFL 0.5",
+            "cells": ni, "faces": nf,
+        },
+        "fp64": {k: fp[k] for k in KEYS if k in fp},
+        "mixed": {k: mx[k] for k in KEYS if k in mx},
+        "mixed_over_fp64": mx["cell_updates_per_s"] / fp["cell_updates_per_s"],
+        "mixed_error_vs_fp64": err,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": f"{dom} kernel of the mixed path",
+            "achieved": mx[dom + "_gbs"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": mx[dom + "_gbs"] / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": mx[dom + "_bytes"],
+            "mean_launch_ms": mx[dom + "_ms"],
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -102,11 +102,13 @@ double teal_gpu_implicit_euler(TealGpu *gpu, double *time, double *residual, dou
 int teal_gpu_run_lserk(TealGpu *gpu, int steps, double *time, double courant, int time_order,
                        int num_stages);
 
-// Per-kernel timing of the dominant (face flux) kernel with HIP events recorded on the
-// library's stream: enable, run, then read the mean duration (ms) and launch count.
+// Per-kernel timing with HIP events recorded on the library's stream around each launch:
+// teal_gpu_timing(gpu, capacity) preallocates events for `capacity` launches per kernel
+// (0 stops timing); teal_gpu_timing_ms returns the mean duration in ms of kernel 0 (gradient
+// + limiter), 1 (face fluxes; split path only) or 2 (the stage kernel: face fluxes in LDS +
+// accumulation + RK update, or the accumulation alone on the split path) and the launch count.
 int teal_gpu_timing(TealGpu *gpu, int capacity);
-double teal_gpu_timing_flux_ms(TealGpu *gpu, int *launches);
-double teal_gpu_timing_grad_ms(TealGpu *gpu, int *launches);
+double teal_gpu_timing_ms(TealGpu *gpu, int kernel, int *launches);
 
 // Device-layout facts for measurement: counts and the algorithmic bytes moved per launch
 // of the gradient/limiter kernel and of the flux kernel (see the byte model in the docs).

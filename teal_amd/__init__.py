@@ -82,8 +82,7 @@ def lib():
         _sig(l, "teal_gpu_implicit_euler", D, VP, DP, DP, D, D, D, D, C.c_int)
         _sig(l, "teal_gpu_run_lserk", C.c_int, VP, C.c_int, DP, D, C.c_int, C.c_int)
         _sig(l, "teal_gpu_timing", C.c_int, VP, C.c_int)
-        _sig(l, "teal_gpu_timing_flux_ms", D, VP, IP)
-        _sig(l, "teal_gpu_timing_grad_ms", D, VP, IP)
+        _sig(l, "teal_gpu_timing_ms", D, VP, C.c_int, IP)
         _sig(l, "teal_gpu_info", C.c_int, VP, LP)
         _lib = l
     return _lib

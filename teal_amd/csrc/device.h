@@ -18,36 +18,53 @@ struct TealGpu;
 template <typename S>
 struct DevData {
     int *d_inc = nullptr, *d_nbr = nullptr;
+    int2 *d_flr = nullptr;
     S *d_w = nullptr, *d_o = nullptr, *d_vol = nullptr, *d_proj = nullptr;
     S *d_area = nullptr, *d_basis = nullptr, *d_offl = nullptr, *d_offr = nullptr,
       *d_corr = nullptr;
     double *d_vol64 = nullptr;
     int *d_bleft = nullptr, *d_bright = nullptr, *d_bface = nullptr, *d_bkind = nullptr;
     S *d_bctx = nullptr;
-    S *d_P[2] = {nullptr, nullptr};
-    S *d_G = nullptr, *d_psi = nullptr, *d_eps2 = nullptr;
+    S *d_P = nullptr;    // [num_cells][5] current state
+    S *d_P2 = nullptr;   // [num_cells][5] the other state buffer of the fused stage kernel
+    S *d_GP = nullptr;   // [num_inner][20]
+    S *d_F = nullptr;    // [num_faces][10] (split path only)
+    S *d_eps2 = nullptr;
+    // per-block face lists of the fused kernel (fv::BlockTopo)
+    int *d_boff = nullptr, *d_bflist = nullptr, *d_bslot = nullptr;
+    int max_block_faces = 0;
+    int fused = 0;  // 1: residual passes use k_flux_block (its face lists fit in LDS and the
+                    // environment does not set TEAL_GPU_SPLIT=1)
 
     int build(const Mesh *mesh, TealGpu *gpu);
+    int build_blocks(const std::vector<int> &inc, int ni);
     void release();
     fv::Geom<S> geom() const;
+    fv::BlockTopo block_topo() const;
+    size_t block_lds() const { return sizeof(S) * 10 * (size_t)max_block_faces; }
 };
 
 // kernel launchers of one precision; instantiated in device.hip (double) and mixed.hip (float)
 template <typename S>
 struct Kern {
-    static void boundary(hipStream_t s, int num, int nc, int nf, const DevData<S> &d, S *P,
-                         double gamma);
+    static void boundary(hipStream_t s, int num, int nf, const DevData<S> &d, double gamma);
     static void grad_limit(hipStream_t s, int lim, int mode, fv::Topo t, fv::Geom<S> g,
-                           const S *P, S *G, S *psi);
-    static int flux(hipStream_t s, int flux, int visc, int order, int mode, fv::Topo t,
-                    fv::Geom<S> g, fv::Phys ph, const S *P, const S *G, const S *psi, S *Pout,
-                    fv::StageArgs a);
+                           const S *P, S *GP);
+    static int face(hipStream_t s, int flux, int visc, int order, fv::Topo t, fv::Geom<S> g,
+                    fv::Phys ph, const S *P, const S *GP, S *F, int *error);
+    static void accum(hipStream_t s, int visc, int conv, int mode, fv::Topo t, fv::Geom<S> g,
+                      fv::Phys ph, const S *F, S *P, fv::StageArgs a);
+    static int flux_block(hipStream_t s, int flux, int visc, int order, int mode, size_t lds,
+                          fv::Topo t, fv::BlockTopo bt, fv::Geom<S> g, fv::Phys ph, const S *P,
+                          const S *GP, S *Pout, fv::StageArgs a);
     static void timestep_u0(hipStream_t s, fv::Topo t, fv::Geom<S> g, fv::Phys ph,
                             const S *P, double *U0, fv::Scalars *sc, int write_u0);
-    static void aos_to_soa(hipStream_t s, const double *in, S *out, int num, int stride, int width,
-                           int out_stride);
-    static void soa_to_aos(hipStream_t s, const S *in, double *out, int num, int stride, int width,
-                           int in_stride);
+    // host layout (reference numbering) <-> device layout (device numbering);
+    // perm[d] = reference index of device row d (null: identity)
+    static void to_device(hipStream_t s, const double *in, S *out, int num, int in_stride,
+                          int width, int out_stride, const int *perm, int soa);
+    static void to_host(hipStream_t s, const S *in, double *out, int num, int in_stride,
+                        int width, int out_stride, const int *perm, int soa);
 };
 
 // a preallocated pool of event pairs, recorded around launches of one kernel
@@ -69,12 +86,13 @@ struct TealGpu {
     int ni = 0, nc = 0, nf = 0, nfi = 0, nfb = 0;
     double sum_volume = 0;
     std::vector<int> periodic_src;
+    std::vector<int> cell_perm, cell_inv, face_perm, face_inv;  // device <-> reference numbering
+    int *d_cell_perm = nullptr;
     DevData<double> d64;
     DevData<float> d32;
     TealGpuConfig cfg{};
     int configured = 0;
     int u0_valid = 0;
-    int cur = 0;
     int nblocks = 0;
     hipStream_t stream = nullptr;
     double *d_stage = nullptr;
@@ -84,7 +102,7 @@ struct TealGpu {
     fv::Scalars *d_scalars = nullptr;
     fv::Scalars *h_scalars = nullptr;
     int timing = 0;
-    EventPool ev_flux, ev_grad;
+    EventPool ev_flux, ev_grad, ev_face;
     Krylov *krylov = nullptr;
 
     fv::Topo topo() const;
@@ -93,7 +111,8 @@ struct TealGpu {
 
 // residual pass helpers shared with krylov.hip
 template <typename S>
-int teal_residual_pass(TealGpu *g, DevData<S> &d, int cur, int mode, double alpha, int last);
+int teal_residual_pass(TealGpu *g, DevData<S> &d, int mode, double alpha, int last);
 template <typename S>
 int teal_timestep_pass(TealGpu *g, DevData<S> &d, int write_u0, double *out);
 void teal_krylov_release(TealGpu *g);
+void teal_device_numbering(TealGpu *g, const Mesh *m);

@@ -8,6 +8,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
+#include <cstdint>
 #include <string>
 #include <vector>
 
@@ -161,6 +163,74 @@ static std::vector<int> periodic_sources(const Mesh *m)
     return src;
 }
 
+// Device numbering (the integer renumbering into the device layout):
+//  - inner cells in Morton (Z-curve) order of their centres, so a wavefront's 64 cells and
+//    their face neighbours occupy a compact region that stays in the XCD's L2;
+//  - ghost cells keep their reference index (they follow the inner cells);
+//  - faces grouped by the device index of their left cell.
+// Each cell's incidence list stays sorted by REFERENCE face index, which is the order the
+// reference accumulates face contributions in, so results do not depend on this numbering.
+static uint64_t spread_bits(uint64_t v)  // 21 bits -> every third bit of 63
+{
+    v &= 0x1fffff;
+    v = (v | (v << 32)) & 0x1f00000000ffffULL;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffULL;
+    v = (v | (v << 8)) & 0x100f00f00f00f00fULL;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ULL;
+    v = (v | (v << 2)) & 0x1249249249249249ULL;
+    return v;
+}
+
+void teal_device_numbering(TealGpu *g, const Mesh *m)
+{
+    const int ni = m->cells.num_inner;
+    const int nc = m->cells.num;
+    const int nf = m->faces.num;
+    double lo[3] = {INFINITY, INFINITY, INFINITY};
+    double hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int c = 0; c < ni; c++) {
+        const double *x = &m->cells.center[c].x;
+        for (int d = 0; d < 3; d++) {
+            lo[d] = fmin(lo[d], x[d]);
+            hi[d] = fmax(hi[d], x[d]);
+        }
+    }
+    std::vector<std::pair<uint64_t, int>> key(ni);
+    for (int c = 0; c < ni; c++) {
+        const double *x = &m->cells.center[c].x;
+        uint64_t k = 0;
+        for (int d = 0; d < 3; d++) {
+            double span = hi[d] - lo[d];
+            double t = (span > 0) ? (x[d] - lo[d]) / span : 0.0;
+            uint64_t q = (uint64_t)llround(t * 2097151.0);
+            k |= spread_bits(q) << d;
+        }
+        key[c] = {k, c};
+    }
+    std::sort(key.begin(), key.end());
+    g->cell_perm.assign(nc, 0);
+    g->cell_inv.assign(nc, 0);
+    for (int d = 0; d < ni; d++) {
+        g->cell_perm[d] = key[d].second;
+        g->cell_inv[key[d].second] = d;
+    }
+    for (int c = ni; c < nc; c++) {
+        g->cell_perm[c] = c;
+        g->cell_inv[c] = c;
+    }
+    std::vector<std::pair<long, int>> fkey(nf);
+    for (int f = 0; f < nf; f++) {
+        fkey[f] = {((long)g->cell_inv[m->faces.cell_idx[f].left] << 32) | f, f};
+    }
+    std::sort(fkey.begin(), fkey.end());
+    g->face_perm.assign(nf, 0);
+    g->face_inv.assign(nf, 0);
+    for (int d = 0; d < nf; d++) {
+        g->face_perm[d] = fkey[d].second;
+        g->face_inv[fkey[d].second] = d;
+    }
+}
+
 template <typename S>
 int DevData<S>::build(const Mesh *m, TealGpu *gpu)
 {
@@ -169,34 +239,39 @@ int DevData<S>::build(const Mesh *m, TealGpu *gpu)
     const int nf = m->faces.num;
     const int nfi = m->faces.num_inner;
     const int nfb = m->faces.off_boundary;
+    const std::vector<int> &cinv = gpu->cell_inv;
+    const std::vector<int> &cperm = gpu->cell_perm;
+    const std::vector<int> &finv = gpu->face_inv;
+    const std::vector<int> &fperm = gpu->face_perm;
     std::vector<int> src = periodic_sources(m);
-    auto resolve = [&](int cell) {
+    auto resolve = [&](int cell) {  // periodic ghost -> its source cell, then device index
         if (cell >= m->cells.off_boundary && src[cell] >= 0) {
-            return src[cell];
+            cell = src[cell];
         }
-        return cell;
+        return cinv[cell];
     };
 
-    // incidences sorted by face index (face order == accumulation order)
+    // incidences: walking faces in reference order keeps each list sorted by reference index
     std::vector<int> cnt(ni, 0);
     std::vector<int> inc((size_t)W * ni, -1), nbr((size_t)W * ni, 0);
     std::vector<double> w((size_t)3 * W * ni, 0.0), o((size_t)3 * W * ni, 0.0);
     for (int f = 0; f < nf; f++) {
         int l = m->faces.cell_idx[f].left;
         int r = m->faces.cell_idx[f].right;
+        int bnd = (f >= nfi && f < nfb);
         for (int side = 0; side < 2; side++) {
-            int c = side ? r : l;
             if (side == 1 && f >= nfi) {
                 break;
             }
+            int c = cinv[side ? r : l];
             int k = cnt[c]++;
             if (k >= W) {
-                teal_gpu_set_error("cell %d has more than %d faces", c, W);
+                teal_gpu_set_error("cell %d has more than %d faces", side ? r : l, W);
                 return -1;
             }
             size_t idx = ((size_t)k * ni) + c;
-            inc[idx] = (f << 1) | side;
-            nbr[idx] = side ? l : resolve(r);
+            inc[idx] = (finv[f] << 2) | (bnd << 1) | side;
+            nbr[idx] = side ? cinv[l] : resolve(r);
             Vector wv = side ? m->faces.weight[f].right : m->faces.weight[f].left;
             Vector ov = side ? m->faces.offset[f].right : m->faces.offset[f].left;
             double wa[3] = {wv.x, wv.y, wv.z};
@@ -208,42 +283,48 @@ int DevData<S>::build(const Mesh *m, TealGpu *gpu)
         }
     }
     std::vector<double> vol(ni), proj((size_t)3 * ni);
-    for (int c = 0; c < ni; c++) {
-        vol[c] = m->cells.volume[c];
-        proj[c] = m->cells.projection[c].x;
-        proj[ni + c] = m->cells.projection[c].y;
-        proj[(2 * ni) + c] = m->cells.projection[c].z;
+    for (int d = 0; d < ni; d++) {
+        int c = cperm[d];
+        vol[d] = m->cells.volume[c];
+        proj[d] = m->cells.projection[c].x;
+        proj[ni + d] = m->cells.projection[c].y;
+        proj[(2 * ni) + d] = m->cells.projection[c].z;
     }
     std::vector<double> area(nf), basis((size_t)9 * nf), offl((size_t)3 * nf), offr((size_t)3 * nf),
         corr((size_t)4 * nf);
-    for (int f = 0; f < nf; f++) {
-        area[f] = m->faces.area[f];
+    std::vector<int2> flr(nf);
+    for (int d = 0; d < nf; d++) {
+        int f = fperm[d];
+        int r = m->faces.cell_idx[f].right;
+        flr[d].x = cinv[m->faces.cell_idx[f].left];
+        flr[d].y = (f >= nfi && f < nfb) ? ~r : resolve(r);
+        area[d] = m->faces.area[f];
         const double *b = &m->faces.basis[f].x.x;
-        for (int k = 0; k < 9; k++) basis[((size_t)k * nf) + f] = b[k];
+        for (int k = 0; k < 9; k++) basis[((size_t)k * nf) + d] = b[k];
         const double *ol = &m->faces.offset[f].left.x;
         const double *orr = &m->faces.offset[f].right.x;
-        for (int d = 0; d < 3; d++) {
-            offl[((size_t)d * nf) + f] = ol[d];
-            offr[((size_t)d * nf) + f] = orr[d];
+        for (int k = 0; k < 3; k++) {
+            offl[((size_t)k * nf) + d] = ol[k];
+            offr[((size_t)k * nf) + d] = orr[k];
         }
-        corr[f] = m->faces.correction[f].unit.x;
-        corr[nf + f] = m->faces.correction[f].unit.y;
-        corr[(2 * (size_t)nf) + f] = m->faces.correction[f].unit.z;
-        corr[(3 * (size_t)nf) + f] = m->faces.correction[f].norm;
+        corr[d] = m->faces.correction[f].unit.x;
+        corr[nf + d] = m->faces.correction[f].unit.y;
+        corr[(2 * (size_t)nf) + d] = m->faces.correction[f].unit.z;
+        corr[(3 * (size_t)nf) + d] = m->faces.correction[f].norm;
     }
-    if (upload(&d_inc, inc) || upload(&d_nbr, nbr) || upload_conv(&d_w, w) ||
+    if (upload(&d_inc, inc) || upload(&d_nbr, nbr) || upload(&d_flr, flr) || upload_conv(&d_w, w) ||
         upload_conv(&d_o, o) || upload_conv(&d_vol, vol) || upload_conv(&d_proj, proj) ||
         upload_conv(&d_area, area) || upload_conv(&d_basis, basis) ||
         upload_conv(&d_offl, offl) || upload_conv(&d_offr, offr) || upload_conv(&d_corr, corr) ||
         upload(&d_vol64, vol)) {
         return -1;
     }
-    // boundary faces
+    // boundary faces: left cell in device numbering, ghost slot at its reference index
     int nb = nfb - nfi;
     std::vector<int> bl(nb), br(nb), bf(nb);
     for (int i = 0; i < nb; i++) {
-        bf[i] = nfi + i;
-        bl[i] = m->faces.cell_idx[nfi + i].left;
+        bf[i] = finv[nfi + i];
+        bl[i] = cinv[m->faces.cell_idx[nfi + i].left];
         br[i] = m->faces.cell_idx[nfi + i].right;
     }
     if (upload(&d_bleft, bl) || upload(&d_bright, br) || upload(&d_bface, bf)) {
@@ -253,27 +334,79 @@ int DevData<S>::build(const Mesh *m, TealGpu *gpu)
     HIPCALL(hipMalloc((void **)&d_bctx, sizeof(S) * 5 * (nb + 1)));
     HIPCALL(hipMemset(d_bkind, 0, sizeof(int) * (nb + 1)));
     HIPCALL(hipMemset(d_bctx, 0, sizeof(S) * 5 * (nb + 1)));
-    // state
-    for (int i = 0; i < 2; i++) {
-        HIPCALL(hipMalloc((void **)&d_P[i], sizeof(S) * 5 * (size_t)nc));
-        HIPCALL(hipMemset(d_P[i], 0, sizeof(S) * 5 * (size_t)nc));
-    }
-    HIPCALL(hipMalloc((void **)&d_G, sizeof(S) * 15 * (size_t)ni));
-    HIPCALL(hipMalloc((void **)&d_psi, sizeof(S) * 5 * (size_t)ni));
-    HIPCALL(hipMemset(d_G, 0, sizeof(S) * 15 * (size_t)ni));
-    HIPCALL(hipMemset(d_psi, 0, sizeof(S) * 5 * (size_t)ni));
+    // state records
+    HIPCALL(hipMalloc((void **)&d_P, sizeof(S) * 5 * (size_t)nc));
+    HIPCALL(hipMemset(d_P, 0, sizeof(S) * 5 * (size_t)nc));
+    HIPCALL(hipMalloc((void **)&d_P2, sizeof(S) * 5 * (size_t)nc));
+    HIPCALL(hipMemset(d_P2, 0, sizeof(S) * 5 * (size_t)nc));
+    HIPCALL(hipMalloc((void **)&d_GP, sizeof(S) * 20 * (size_t)ni));
+    HIPCALL(hipMemset(d_GP, 0, sizeof(S) * 20 * (size_t)ni));
+    HIPCALL(hipMalloc((void **)&d_F, sizeof(S) * 10 * (size_t)(nf + 1)));
+    HIPCALL(hipMemset(d_F, 0, sizeof(S) * 10 * (size_t)(nf + 1)));
     HIPCALL(hipMalloc((void **)&d_eps2, sizeof(S) * (size_t)ni));
     HIPCALL(hipMemset(d_eps2, 0, sizeof(S) * (size_t)ni));
-    (void)gpu;
+    return build_blocks(inc, ni);
+}
+
+// Face lists of the fused stage kernel: block b owns device cells [b*BLOCK, (b+1)*BLOCK); its
+// list holds every face with an incidence in the block, once, in device face order, and
+// slot[k][c] = (list position << 1) | side for incidence k of cell c (-1: none).
+template <typename S>
+int DevData<S>::build_blocks(const std::vector<int> &inc, int ni)
+{
+    const int nblk = (ni + BLOCK - 1) / BLOCK;
+    std::vector<int> off(nblk + 1, 0), list, slot((size_t)W * ni, -1);
+    std::vector<int> faces;
+    max_block_faces = 0;
+    for (int b = 0; b < nblk; b++) {
+        int c0 = b * BLOCK;
+        int c1 = std::min(ni, c0 + BLOCK);
+        faces.clear();
+        for (int c = c0; c < c1; c++) {
+            for (int k = 0; k < W; k++) {
+                int e = inc[((size_t)k * ni) + c];
+                if (e >= 0) {
+                    faces.push_back(e >> 2);
+                }
+            }
+        }
+        std::sort(faces.begin(), faces.end());
+        faces.erase(std::unique(faces.begin(), faces.end()), faces.end());
+        for (int c = c0; c < c1; c++) {
+            for (int k = 0; k < W; k++) {
+                size_t idx = ((size_t)k * ni) + c;
+                int e = inc[idx];
+                if (e >= 0) {
+                    int pos = (int)(std::lower_bound(faces.begin(), faces.end(), e >> 2) - faces.begin());
+                    slot[idx] = (pos << 1) | (e & 1);
+                }
+            }
+        }
+        list.insert(list.end(), faces.begin(), faces.end());
+        off[b + 1] = (int)list.size();
+        max_block_faces = std::max(max_block_faces, (int)faces.size());
+    }
+    if (list.empty()) {
+        list.push_back(0);
+    }
+    if (upload(&d_boff, off) || upload(&d_bflist, list) || upload(&d_bslot, slot)) {
+        return -1;
+    }
     return 0;
+}
+
+template <typename S>
+BlockTopo DevData<S>::block_topo() const
+{
+    return BlockTopo{d_boff, d_bflist, d_bslot, max_block_faces};
 }
 
 template <typename S>
 void DevData<S>::release()
 {
-    void *ptrs[] = {d_inc, d_nbr, d_w, d_o, d_vol, d_proj, d_area, d_basis, d_offl, d_offr,
-                    d_corr, d_vol64, d_bleft, d_bright, d_bface, d_bkind, d_bctx, d_P[0],
-                    d_P[1], d_G, d_psi, d_eps2};
+    void *ptrs[] = {d_inc, d_nbr, d_flr, d_w, d_o, d_vol, d_proj, d_area, d_basis, d_offl,
+                    d_offr, d_corr, d_vol64, d_bleft, d_bright, d_bface, d_bkind, d_bctx,
+                    d_P, d_P2, d_GP, d_F, d_eps2, d_boff, d_bflist, d_bslot};
     for (void *p : ptrs) {
         if (p) {
             (void)hipFree(p);
@@ -292,8 +425,6 @@ template struct DevData<float>;
 
 
 // ------------------------------------------------------------------ TealGpu
-static int grid_of(int n) { return (n + BLOCK - 1) / BLOCK; }
-
 extern "C" TealGpu *teal_gpu_create(const Mesh *mesh, int precision)
 {
     if (!mesh || !mesh->faces.area) {
@@ -319,7 +450,22 @@ extern "C" TealGpu *teal_gpu_create(const Mesh *mesh, int precision)
     g->nfb = mesh->faces.off_boundary;
     g->sum_volume = mesh->cells.sum_volume;
     g->periodic_src = periodic_sources(mesh);
+    teal_device_numbering(g, mesh);
     int rc = (precision == TEAL_FP64) ? g->d64.build(mesh, g) : g->d32.build(mesh, g);
+    if (rc == 0) {  // the fused stage kernel needs a block's face list in LDS
+        int dev = 0, lds_max = 0;
+        const char *split = getenv("TEAL_GPU_SPLIT");
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) {
+            lds_max = 0;
+        }
+        int want = !(split && atoi(split) != 0);
+        g->d64.fused = want && g->d64.block_lds() > 0 && g->d64.block_lds() <= (size_t)lds_max;
+        g->d32.fused = want && g->d32.block_lds() > 0 && g->d32.block_lds() <= (size_t)lds_max;
+    }
+    if (rc == 0 && upload(&g->d_cell_perm, g->cell_perm)) {
+        rc = -1;
+    }
     if (rc == 0 && hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess) {
         teal_gpu_set_error("teal_gpu_create: hipStreamCreate failed");
         rc = -1;
@@ -360,7 +506,7 @@ extern "C" void teal_gpu_destroy(TealGpu *g)
     }
     g->d64.release();
     g->d32.release();
-    void *ptrs[] = {g->d_stage, g->d_U0, g->d_R, g->d_partials, g->d_scalars};
+    void *ptrs[] = {g->d_stage, g->d_U0, g->d_R, g->d_partials, g->d_scalars, g->d_cell_perm};
     for (void *p : ptrs) {
         if (p) {
             (void)hipFree(p);
@@ -371,6 +517,7 @@ extern "C" void teal_gpu_destroy(TealGpu *g)
     }
     g->ev_flux.release();
     g->ev_grad.release();
+    g->ev_face.release();
     teal_krylov_release(g);
     if (g->stream) {
         (void)hipStreamDestroy(g->stream);
@@ -411,9 +558,10 @@ static int configure_impl(TealGpu *g, DevData<S> &d, const TealGpuConfig *cfg)
     HIPCALL(hipMemcpy(d.d_bctx, ctx.data(), sizeof(S) * 5 * (nb + 1), hipMemcpyHostToDevice));
     if (cfg->limiter == VENKATAKRISHNAN_ID) {
         std::vector<S> e2(g->ni);
-        for (int c = 0; c < g->ni; c++) {  // src/equations/limiter.c:8-21
+        for (int d = 0; d < g->ni; d++) {  // src/equations/limiter.c:8-21
+            int c = g->cell_perm[d];
             double v = cfg->coefficient * cbrt(m->cells.volume[c]);
-            e2[c] = cfg->eps2 ? (S)cfg->eps2[c] : (S)(v * v * v);
+            e2[d] = cfg->eps2 ? (S)cfg->eps2[c] : (S)(v * v * v);
         }
         HIPCALL(hipMemcpy(d.d_eps2, e2.data(), sizeof(S) * g->ni, hipMemcpyHostToDevice));
     }
@@ -454,30 +602,31 @@ extern "C" int teal_gpu_configure(TealGpu *g, const TealGpuConfig *cfg)
 // ------------------------------------------------------------------ kernel launches
 Topo TealGpu::topo() const
 {
-    const int *inc = (precision == TEAL_FP64) ? d64.d_inc : d32.d_inc;
-    const int *nbr = (precision == TEAL_FP64) ? d64.d_nbr : d32.d_nbr;
-    return Topo{ni, nc, nf, nfi, nfb, inc, nbr};
+    if (precision == TEAL_FP64) {
+        return Topo{ni, nc, nf, d64.d_inc, d64.d_nbr, d64.d_flr};
+    }
+    return Topo{ni, nc, nf, d32.d_inc, d32.d_nbr, d32.d_flr};
 }
 
 Phys TealGpu::phys() const { return Phys{cfg.gamma, cfg.mu, cfg.prandtl, cfg.navierstokes}; }
 
-// one residual evaluation on state P[cur]: ghosts, gradient + limiter, fluxes.
-// mode 0 writes R; mode 1 is an RK stage writing P[1-cur] (last: + U0, dt, residual)
+// one residual evaluation of the state in P: ghosts, gradient + limiter, face fluxes,
+// accumulation.  mode 0 writes R; mode 1 is an RK stage updating P (last: + U0, dt, residual)
 template <typename S>
-int teal_residual_pass(TealGpu *g, DevData<S> &d, int cur, int mode, double alpha, int last)
+int teal_residual_pass(TealGpu *g, DevData<S> &d, int mode, double alpha, int last)
 {
     const int nb = g->nfb - g->nfi;
     if (nb > 0) {
-        Kern<S>::boundary(g->stream, nb, g->nc, g->nf, d, d.d_P[cur], g->cfg.gamma);
+        Kern<S>::boundary(g->stream, nb, g->nf, d, g->cfg.gamma);
     }
     if (g->cfg.space_order == 2) {
         bool timed = g->timing && g->ev_grad.begin(g->stream);
-        Kern<S>::grad_limit(g->stream, g->cfg.limiter, 0, g->topo(), d.geom(), d.d_P[cur], d.d_G,
-                            d.d_psi);
+        Kern<S>::grad_limit(g->stream, g->cfg.limiter, 0, g->topo(), d.geom(), d.d_P, d.d_GP);
         if (timed) {
             g->ev_grad.end(g->stream);
         }
     }
+    const int conv = g->cfg.flux != TEAL_FLUX_NONE;
     StageArgs a;
     a.R = g->d_R;
     a.U0 = g->d_U0;
@@ -487,40 +636,67 @@ int teal_residual_pass(TealGpu *g, DevData<S> &d, int cur, int mode, double alph
     a.partials = g->d_partials;
     a.keep_u = (g->precision == TEAL_MIXED);
     int kmode = (mode == 0) ? 0 : (last ? 2 : 1);
+    if (d.fused) {  // one kernel: face fluxes in LDS, accumulation, stage update into d_P2
+        bool timed = g->timing && g->ev_flux.begin(g->stream);
+        int rc = Kern<S>::flux_block(g->stream, g->cfg.flux, g->cfg.viscous, g->cfg.space_order, kmode,
+                                     d.block_lds(), g->topo(), d.block_topo(), d.geom(), g->phys(),
+                                     d.d_P, d.d_GP, d.d_P2, a);
+        if (timed) {
+            g->ev_flux.end(g->stream);
+        }
+        if (rc) {
+            return rc;
+        }
+        HIPCALL(hipGetLastError());
+        if (kmode != 0) {  // neighbouring blocks read the old state: the stage writes the other buffer
+            std::swap(d.d_P, d.d_P2);
+        }
+        return 0;
+    }
+    if (conv || g->cfg.viscous) {
+        bool timed = g->timing && g->ev_face.begin(g->stream);
+        int rc = Kern<S>::face(g->stream, g->cfg.flux, g->cfg.viscous, g->cfg.space_order, g->topo(),
+                               d.geom(), g->phys(), d.d_P, d.d_GP, d.d_F, &g->d_scalars->error);
+        if (timed) {
+            g->ev_face.end(g->stream);
+        }
+        if (rc) {
+            return rc;
+        }
+    }
     bool timed = g->timing && g->ev_flux.begin(g->stream);
-    int rc = Kern<S>::flux(g->stream, g->cfg.flux, g->cfg.viscous, g->cfg.space_order, kmode,
-                           g->topo(), d.geom(), g->phys(), d.d_P[cur], d.d_G, d.d_psi,
-                           d.d_P[1 - cur], a);
+    Kern<S>::accum(g->stream, g->cfg.viscous, conv, kmode, g->topo(), d.geom(), g->phys(), d.d_F, d.d_P,
+                   a);
     if (timed) {
         g->ev_flux.end(g->stream);
-    }
-    if (rc) {
-        return rc;
     }
     HIPCALL(hipGetLastError());
     return 0;
 }
 
-template int teal_residual_pass<double>(TealGpu *, DevData<double> &, int, int, double, int);
-template int teal_residual_pass<float>(TealGpu *, DevData<float> &, int, int, double, int);
+template int teal_residual_pass<double>(TealGpu *, DevData<double> &, int, double, int);
+template int teal_residual_pass<float>(TealGpu *, DevData<float> &, int, double, int);
 
+// host primitive rows (reference numbering) -> device P records
 template <typename S>
 static int upload_primitive(TealGpu *g, DevData<S> &d, const double *primitive)
 {
     size_t bytes = sizeof(double) * 5 * (size_t)g->nc;
     HIPCALL(hipMemcpyAsync(g->d_stage, primitive, bytes, hipMemcpyHostToDevice, g->stream));
-    Kern<S>::aos_to_soa(g->stream, g->d_stage, d.d_P[g->cur], g->nc, 5, 5, g->nc);
+    Kern<S>::to_device(g->stream, g->d_stage, d.d_P, g->nc, 5, 5, 5, g->d_cell_perm, 0);
     HIPCALL(hipGetLastError());
     return 0;
 }
 
+// device rows (records or [width][num] arrays) -> host rows of `host_stride` doubles
 template <typename S>
-static int download_soa(TealGpu *g, const S *src, int num, int stride, int width, int src_stride,
-                        double *host)
+static int download_rows(TealGpu *g, const S *src, int num, int src_stride, int width, int soa,
+                         int host_stride, double *host)
 {
-    Kern<S>::soa_to_aos(g->stream, src, g->d_stage, num, stride, width, src_stride);
+    Kern<S>::to_host(g->stream, src, g->d_stage, num, src_stride, width, host_stride, g->d_cell_perm,
+                     soa);
     HIPCALL(hipGetLastError());
-    HIPCALL(hipMemcpyAsync(host, g->d_stage, sizeof(double) * stride * (size_t)num,
+    HIPCALL(hipMemcpyAsync(host, g->d_stage, sizeof(double) * host_stride * (size_t)num,
                            hipMemcpyDeviceToHost, g->stream));
     HIPCALL(hipStreamSynchronize(g->stream));
     return 0;
@@ -555,22 +731,18 @@ template <typename S>
 static int derivative_impl(TealGpu *g, DevData<S> &d, double *primitive, double *derivative)
 {
     if (upload_primitive(g, d, primitive)) return -1;
-    if (teal_residual_pass(g, d, g->cur, 0, 0.0, 0)) return -1;
+    if (teal_residual_pass(g, d, 0, 0.0, 0)) return -1;
     // ghost states back to the caller (boundary ghosts computed on the device)
     int nb = g->nfb - g->nfi;
     if (nb > 0) {
         std::vector<double> tmp((size_t)5 * g->nc);
-        if (download_soa(g, d.d_P[g->cur], g->nc, 5, 5, g->nc, tmp.data())) return -1;
+        if (download_rows(g, d.d_P, g->nc, 5, 5, 0, 5, tmp.data())) return -1;
         for (int c = g->ni; c < g->mesh->cells.off_boundary; c++) {
             memcpy(&primitive[(size_t)c * 5], &tmp[(size_t)c * 5], sizeof(double) * 5);
         }
     }
     fill_periodic_ghosts(g, primitive, 5);
-    Kern<double>::soa_to_aos(g->stream, g->d_R, g->d_stage, g->ni, 5, 5, g->ni);
-    HIPCALL(hipGetLastError());
-    HIPCALL(hipMemcpyAsync(derivative, g->d_stage, sizeof(double) * 5 * (size_t)g->ni,
-                           hipMemcpyDeviceToHost, g->stream));
-    HIPCALL(hipStreamSynchronize(g->stream));
+    if (download_rows(g, (const double *)g->d_R, g->ni, 5, 5, 1, 5, derivative)) return -1;
     g->u0_valid = 0;
     return 0;
 }
@@ -588,10 +760,10 @@ static int boundary_impl(TealGpu *g, DevData<S> &d, double *primitive)
     if (upload_primitive(g, d, primitive)) return -1;
     int nb = g->nfb - g->nfi;
     if (nb > 0) {
-        Kern<S>::boundary(g->stream, nb, g->nc, g->nf, d, d.d_P[g->cur], g->cfg.gamma);
+        Kern<S>::boundary(g->stream, nb, g->nf, d, g->cfg.gamma);
         HIPCALL(hipGetLastError());
         std::vector<double> tmp((size_t)5 * g->nc);
-        if (download_soa(g, d.d_P[g->cur], g->nc, 5, 5, g->nc, tmp.data())) return -1;
+        if (download_rows(g, d.d_P, g->nc, 5, 5, 0, 5, tmp.data())) return -1;
         for (int c = g->ni; c < g->mesh->cells.off_boundary; c++) {
             memcpy(&primitive[(size_t)c * 5], &tmp[(size_t)c * 5], sizeof(double) * 5);
         }
@@ -613,24 +785,23 @@ static int gradient_impl(TealGpu *g, DevData<S> &d, double *primitive, double *g
     if (upload_primitive(g, d, primitive)) return -1;
     int lim = (mode == 0) ? 0 : g->cfg.limiter;
     if (mode == 1) {  // limiter on the caller's gradient
-        size_t bytes = sizeof(double) * 15 * (size_t)g->nc;
-        HIPCALL(hipMemcpyAsync(g->d_stage, gradient, bytes, hipMemcpyHostToDevice, g->stream));
-        Kern<S>::aos_to_soa(g->stream, g->d_stage, d.d_G, g->ni, 15, 15, g->ni);
         if (lim == 0) {
             teal_gpu_set_error("invalid limiter kind (0)");
             return -1;
         }
+        size_t bytes = sizeof(double) * 15 * (size_t)g->nc;
+        HIPCALL(hipMemcpyAsync(g->d_stage, gradient, bytes, hipMemcpyHostToDevice, g->stream));
+        Kern<S>::to_device(g->stream, g->d_stage, d.d_GP, g->ni, 15, 15, 20, g->d_cell_perm, 0);
     }
-    Kern<S>::grad_limit(g->stream, lim, mode, g->topo(), d.geom(), d.d_P[g->cur], d.d_G, d.d_psi);
+    Kern<S>::grad_limit(g->stream, lim, mode, g->topo(), d.geom(), d.d_P, d.d_GP);
     HIPCALL(hipGetLastError());
-    // limited gradient = G * psi (src/equations/limiter.c:61-63)
-    std::vector<double> G((size_t)15 * g->ni), psi((size_t)5 * g->ni);
-    if (download_soa(g, d.d_G, g->ni, 15, 15, g->ni, G.data())) return -1;
-    if (mode == 1 && download_soa(g, d.d_psi, g->ni, 5, 5, g->ni, psi.data())) return -1;
+    // rows of 20: gradient (15) then psi (5); the limited gradient is G * psi (limiter.c:61-63)
+    std::vector<double> GP((size_t)20 * g->ni);
+    if (download_rows(g, d.d_GP, g->ni, 20, 20, 0, 20, GP.data())) return -1;
     for (int c = 0; c < g->ni; c++) {
+        const double *row = &GP[(size_t)c * 20];
         for (int i = 0; i < 15; i++) {
-            double v = G[((size_t)c * 15) + i];
-            gradient[((size_t)c * 15) + i] = (mode == 1) ? v * psi[((size_t)c * 5) + (i / 3)] : v;
+            gradient[((size_t)c * 15) + i] = (mode == 1) ? row[i] * row[15 + (i / 3)] : row[i];
         }
     }
     if (mode == 0) {
@@ -663,8 +834,8 @@ template <typename S>
 int teal_timestep_pass(TealGpu *g, DevData<S> &d, int write_u0, double *out)
 {
     hipLaunchKernelGGL(k_reset_dt, dim3(1), dim3(1), 0, g->stream, g->d_scalars);
-    Kern<S>::timestep_u0(g->stream, g->topo(), d.geom(), g->phys(), d.d_P[g->cur], g->d_U0,
-                         g->d_scalars, write_u0);
+    Kern<S>::timestep_u0(g->stream, g->topo(), d.geom(), g->phys(), d.d_P, g->d_U0, g->d_scalars,
+                         write_u0);
     HIPCALL(hipGetLastError());
     if (out) {
         HIPCALL(hipMemcpyAsync(g->h_scalars, g->d_scalars, sizeof(Scalars), hipMemcpyDeviceToHost,
@@ -695,7 +866,7 @@ extern "C" int teal_gpu_residual(TealGpu *g, const double *derivative, double *r
     }
     HIPCALL(hipMemcpyAsync(g->d_stage, derivative, sizeof(double) * 5 * (size_t)g->ni,
                            hipMemcpyHostToDevice, g->stream));
-    Kern<double>::aos_to_soa(g->stream, g->d_stage, g->d_R, g->ni, 5, 5, g->ni);
+    Kern<double>::to_device(g->stream, g->d_stage, g->d_R, g->ni, 5, 5, 5, g->d_cell_perm, 1);
     const double *vol64 = (g->precision == TEAL_FP64) ? g->d64.d_vol64 : g->d32.d_vol64;
     hipLaunchKernelGGL(k_residual_partials, dim3(g->nblocks), dim3(BLOCK), 0, g->stream, g->d_R,
                        vol64, g->ni, g->d_partials);
@@ -725,8 +896,8 @@ extern "C" int teal_gpu_download(TealGpu *g, double *primitive)
 {
     if (check_ready(g)) return -1;
     int rc = (g->precision == TEAL_FP64)
-                 ? download_soa(g, g->d64.d_P[g->cur], g->nc, 5, 5, g->nc, primitive)
-                 : download_soa(g, g->d32.d_P[g->cur], g->nc, 5, 5, g->nc, primitive);
+                 ? download_rows(g, g->d64.d_P, g->nc, 5, 5, 0, 5, primitive)
+                 : download_rows(g, g->d32.d_P, g->nc, 5, 5, 0, 5, primitive);
     if (rc) return rc;
     fill_periodic_ghosts(g, primitive, 5);
     return 0;
@@ -772,8 +943,7 @@ static int explicit_step(TealGpu *g, DevData<S> &d, double max_step, double cour
     hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, g->stream, g->d_scalars, courant,
                        max_step);
     for (int s = 0; s < num_stages; s++) {
-        if (teal_residual_pass(g, d, g->cur, 1, alpha[s + 1], s == num_stages - 1)) return -1;
-        g->cur = 1 - g->cur;
+        if (teal_residual_pass(g, d, 1, alpha[s + 1], s == num_stages - 1)) return -1;
     }
     hipLaunchKernelGGL(k_step_end, dim3(1), dim3(BLOCK), 0, g->stream, g->d_scalars,
                        (const double *)g->d_partials, g->nblocks, g->sum_volume, 1);
@@ -911,21 +1081,21 @@ extern "C" int teal_gpu_timing(TealGpu *g, int capacity)
     if (!g) return -1;
     g->timing = capacity > 0;
     if (capacity > 0) {
-        if (g->ev_flux.reserve(capacity) || g->ev_grad.reserve(capacity)) return -1;
+        if (g->ev_grad.reserve(capacity) || g->ev_face.reserve(capacity) ||
+            g->ev_flux.reserve(capacity)) {
+            return -1;
+        }
     }
     return 0;
 }
 
-extern "C" double teal_gpu_timing_flux_ms(TealGpu *g, int *launches)
+// mean duration (ms) of the timed launches of kernel 0 (gradient + limiter), 1 (face fluxes)
+// or 2 (accumulation + RK update)
+extern "C" double teal_gpu_timing_ms(TealGpu *g, int kernel, int *launches)
 {
-    if (launches) *launches = g->ev_flux.used;
-    return g->ev_flux.used ? g->ev_flux.total_ms() / g->ev_flux.used : 0.0;
-}
-
-extern "C" double teal_gpu_timing_grad_ms(TealGpu *g, int *launches)
-{
-    if (launches) *launches = g->ev_grad.used;
-    return g->ev_grad.used ? g->ev_grad.total_ms() / g->ev_grad.used : 0.0;
+    EventPool *pool = (kernel == 0) ? &g->ev_grad : (kernel == 1) ? &g->ev_face : &g->ev_flux;
+    if (launches) *launches = pool->used;
+    return pool->used ? pool->total_ms() / pool->used : 0.0;
 }
 
 extern "C" int teal_gpu_info(TealGpu *g, long *counts)

@@ -260,7 +260,7 @@ typedef struct {
     TealGpu *gpu;
     int precision;
     TealGpuConfig cfg;  // last configuration handed to the device
-    double *eps2;       // parameter pointer it was built from
+    const double *eps2; // parameter pointer it was built from
     int *bc;
     double *ctx;
 } Device;

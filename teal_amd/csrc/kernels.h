@@ -1,25 +1,36 @@
 // kernels.h -- HIP kernels of the residual path, templated on the storage/arithmetic type S
-// of states, gradients and geometry (double: FP64 path; float: mixed path).
+// of states, gradients, geometry and face fluxes (double: FP64 path; float: mixed path).
 //
-// Device layout (all structure-of-arrays, one array per component):
-//   P   [5][num_cells]   primitive {rho,u,v,w,p}; boundary-ghost slots hold BC states,
-//                        periodic ghosts are never stored: they resolve to their source cell
-//   G   [15][num_inner]  LSQ gradient, component (var*3 + xyz)
-//   psi [5][num_inner]   limiter factor (1 when no limiter)
-//   U0  [5][num_inner]   FP64 conserved state at the start of the RK step
-//   incidences, slot-major [W][num_inner]: inc = face<<1 | side (side 1: cell is the
-//   face's right cell), sorted by face index so the per-cell accumulation order equals
-//   the reference's face-order scatter (src/equations/derivative.c:116-224).
-//   per-incidence geometry [3][W][num_inner]: LSQ weight and face offset of that side
-//   face geometry [..][num_faces]: area, basis (9), offset left/right (3+3), correction (4)
+// Device layout.  Cells and faces are stored in the device numbering (inner cells in Morton
+// order, faces grouped by their left cell; see teal_device_numbering):
+//   P    2x [num_cells][5] primitive {rho,u,v,w,p} records (read / written alternately by
+//                         the RK stages); boundary-ghost rows hold the BC
+//                         states, periodic ghosts are never stored (they resolve to their
+//                         source cell)
+//   GP   [num_inner][20]  LSQ gradient (var*3 + xyz, 15 values) then limiter factor psi (5)
+//   F    [num_faces][NF]  face contributions F*A: viscous (5, when present) then convective (5)
+//   U0   [5][num_inner]   FP64 conserved state at the start of the RK step
+//   incidences, slot-major [W][num_inner]: inc = face<<2 | boundary<<1 | side (side 1: the cell
+//   is the face's right cell), slots sorted by the REFERENCE face index, so each cell sums
+//   its face contributions in the order the reference scatters them
+//   (src/equations/derivative.c:116-224); per-incidence LSQ weight and face offset [3][W][ni]
+//   face geometry, structure of arrays [..][num_faces]: area, basis (9), offset l/r (3+3),
+//   correction (4); face cells [num_faces][2] (right < 0: ~ghost of a boundary face)
+// Records (P, GP, F) are arrays of structures because they are gathered through cell and
+// face indices: one record is one or two cache lines instead of one line per component.
 //
-// Kernels (one thread per inner cell, owner-gather, no atomics on state):
-//   k_boundary      ghost states of boundary faces (src/equations/boundary.c:5-43)
-//   k_grad_limit    LSQ gradient (gradient.c:7-58) fused with the limiter (limiter.c:23-123)
-//   k_flux          viscous then convective face fluxes in face order, finalize
-//                   (derivative.c:91-267) and, in RK mode, the stage update + cons2prim
-//                   (src/simulation/advance.c:101-109); the last stage also produces the
-//                   next step's U0, the dt minimum and residual partial sums
+// Kernels per residual evaluation:
+//   k_boundary   ghost states of boundary faces (src/equations/boundary.c:5-43)
+//   k_grad_limit LSQ gradient (gradient.c:7-58) fused with the limiter (limiter.c:23-123);
+//                one thread per inner cell
+//   k_flux_block one workgroup per block of 256 device cells: the viscous and convective flux
+//                of every face touching the block (derivative.c:91-227) into LDS, then per
+//                cell the viscous contributions in face order, the convective ones, R = -R/V
+//                (derivative.c:229-267) and, in RK mode, the stage update + cons2prim
+//                (src/simulation/advance.c:101-109); the last stage also produces the next
+//                step's U0, the dt minimum and residual partial sums
+//   k_face + k_accum  the same work as two kernels through a face flux array in HBM, used
+//                when a block's face list does not fit in LDS
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -33,9 +44,10 @@ constexpr int BLOCK = 256;  // 4 wavefronts of 64
 constexpr int NUM_XCD = 8;
 
 struct Topo {
-    int ni, nc, nf, nfi, nfb;
-    const int *__restrict__ inc;  // [W][ni]
-    const int *__restrict__ nbr;  // [W][ni] resolved neighbour cell
+    int ni, nc, nf;
+    const int *__restrict__ inc;   // [W][ni]
+    const int *__restrict__ nbr;   // [W][ni] neighbour cell (device numbering / ghost row)
+    const int2 *__restrict__ flr;  // [nf] left, right (~ghost for boundary faces)
 };
 
 template <typename S>
@@ -68,8 +80,8 @@ struct Scalars {
 };
 
 // XCD-aware block order: the hardware deals workgroups round-robin to the 8 XCDs; this
-// bijection gives every XCD one contiguous range of cells so face neighbours (close in
-// the BFS numbering) share that XCD's L2.
+// bijection gives every XCD one contiguous range of rows so neighbouring rows (close in the
+// Morton numbering) share that XCD's L2.
 __device__ inline int xcd_block()
 {
     int nb = gridDim.x;
@@ -117,59 +129,60 @@ struct Acc<float> {
     __device__ double get(int i) const { return (double)hi[i] + (double)lo[i]; }
 };
 
+template <typename S>
+__device__ inline void load_rec(S *dst, const S *__restrict__ src, long row, int width)
+{
+    for (int v = 0; v < width; v++) dst[v] = src[(row * width) + v];
+}
+
 // ---------------------------------------------------------------- boundary ghosts
 template <typename S>
-__global__ __launch_bounds__(BLOCK) void k_boundary(int num, int nc, int nf,
-                                                    const int *__restrict__ bleft,
+__global__ __launch_bounds__(BLOCK) void k_boundary(int num, int nf, const int *__restrict__ bleft,
                                                     const int *__restrict__ bright,
                                                     const int *__restrict__ bface,
                                                     const int *__restrict__ bkind,
                                                     const S *__restrict__ bctx,
                                                     const S *__restrict__ basis, S *P,
-                                                    double gamma)
+                                                    S *P2, double gamma)
 {
     int i = (blockIdx.x * BLOCK) + threadIdx.x;
     if (i >= num) {
         return;
     }
-    int l = bleft[i];
-    int r = bright[i];
     int f = bface[i];
     S b9[9];
     for (int k = 0; k < 9; k++) b9[k] = basis[(k * nf) + f];
     S in[5], out[5], ctx[5];
-    for (int v = 0; v < 5; v++) {
-        in[v] = P[(v * nc) + l];
-        ctx[v] = bctx[(i * 5) + v];
-    }
+    load_rec(in, P, bleft[i], 5);
+    for (int v = 0; v < 5; v++) ctx[v] = bctx[(i * 5) + v];
     boundary<S>(bkind[i], out, in, (S)gamma, m3(b9), ctx);
-    for (int v = 0; v < 5; v++) P[(v * nc) + r] = out[v];
+    long r = bright[i];
+    for (int v = 0; v < 5; v++) P[(r * 5) + v] = out[v];
+    if (P2) {  // the other state buffer: the stage writes its inner rows only
+        for (int v = 0; v < 5; v++) P2[(r * 5) + v] = out[v];
+    }
 }
 
 // ---------------------------------------------------------------- gradient + limiter
-// MODE 0: compute the LSQ gradient then limit; MODE 1: limiter only on a given gradient.
+// MODE 0: compute the LSQ gradient then limit; MODE 1: limiter only on the gradient in GP.
 template <typename S, int LIM, int MODE>
-__global__ __launch_bounds__(BLOCK) void k_grad_limit(Topo t, Geom<S> g,
-                                                      const S *__restrict__ P,
-                                                      S *__restrict__ G, S *__restrict__ psi)
+__global__ __launch_bounds__(BLOCK) void k_grad_limit(Topo t, Geom<S> g, const S *__restrict__ P,
+                                                      S *__restrict__ GP)
 {
     int c = (xcd_block() * BLOCK) + threadIdx.x;
     if (c >= t.ni) {
         return;
     }
     const int ni = t.ni;
-    const int nc = t.nc;
     S pc[5], mn[5], mx[5];
-    for (int v = 0; v < 5; v++) {
-        pc[v] = P[(v * nc) + c];
-        mn[v] = mx[v] = pc[v];
-    }
+    load_rec(pc, P, c, 5);
+    for (int v = 0; v < 5; v++) mn[v] = mx[v] = pc[v];
     S gr[15];
     if (MODE == 0) {
         for (int i = 0; i < 15; i++) gr[i] = 0;
     }
     else {
-        for (int i = 0; i < 15; i++) gr[i] = G[(i * ni) + c];
+        for (int i = 0; i < 15; i++) gr[i] = GP[((long)c * 20) + i];
     }
 #pragma unroll
     for (int k = 0; k < W; k++) {
@@ -182,12 +195,13 @@ __global__ __launch_bounds__(BLOCK) void k_grad_limit(Topo t, Geom<S> g,
         S wx = g.w[(((0 * W) + k) * ni) + c];
         S wy = g.w[(((1 * W) + k) * ni) + c];
         S wz = g.w[(((2 * W) + k) * ni) + c];
+        S pn[5];
+        load_rec(pn, P, n, 5);
         for (int v = 0; v < 5; v++) {
-            S pn = P[(v * nc) + n];
-            mn[v] = tmin(mn[v], pn);
-            mx[v] = tmax(mx[v], pn);
+            mn[v] = tmin(mn[v], pn[v]);
+            mx[v] = tmax(mx[v], pn[v]);
             if (MODE == 0) {
-                S diff = side ? (pc[v] - pn) : (pn - pc[v]);  // P[right] - P[left]
+                S diff = side ? (pc[v] - pn[v]) : (pn[v] - pc[v]);  // P[right] - P[left]
                 S s = side ? -diff : diff;
                 gr[(v * 3) + 0] += s * wx;
                 gr[(v * 3) + 1] += s * wy;
@@ -224,13 +238,129 @@ __global__ __launch_bounds__(BLOCK) void k_grad_limit(Topo t, Geom<S> g,
             }
         }
     }
+    S *out = &GP[(long)c * 20];
     if (MODE == 0) {
-        for (int i = 0; i < 15; i++) G[(i * ni) + c] = gr[i];
+        for (int i = 0; i < 15; i++) out[i] = gr[i];
     }
-    for (int v = 0; v < 5; v++) psi[(v * ni) + c] = ps[v];
+    for (int v = 0; v < 5; v++) out[15 + v] = ps[v];
 }
 
-// ---------------------------------------------------------------- face fluxes + update
+// ---------------------------------------------------------------- face fluxes
+// Viscous (fv) and convective (fc) contribution F*A of face f.
+template <typename S, int FLUX, int VISC, int ORDER>
+__device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph,
+                                 const S *__restrict__ P, const S *__restrict__ GP, int f, S *fv,
+                                 S *fc, int *err)
+{
+    const int nf = t.nf;
+    const S gamma = (S)ph.gamma;
+    int2 lr = t.flr[f];
+    const int l = lr.x;
+    const bool bnd = lr.y < 0;
+    const int r = bnd ? ~lr.y : lr.y;
+    S pl[5], pr[5];
+    load_rec(pl, P, l, 5);
+    load_rec(pr, P, r, 5);
+    S gl[20], gr[20];
+    if (ORDER == 2) {
+        load_rec(gl, GP, l, 20);
+        if (!bnd) {
+            load_rec(gr, GP, r, 20);
+        }
+    }
+    const S area = g.area[f];
+    if (VISC) {
+        const S mu = (S)ph.mu;
+        const S prl = (S)ph.prandtl;
+        S pm[5];
+        for (int v = 0; v < 5; v++) pm[v] = (pl[v] + pr[v]) / 2;
+        V3<S> gm[5];
+        for (int v = 0; v < 5; v++) {
+            if (bnd) {  // G[left] only (derivative.c:129-139)
+                gm[v] = v3(gl[(v * 3) + 0], gl[(v * 3) + 1], gl[(v * 3) + 2]);
+            }
+            else {
+                gm[v] = v3((gl[(v * 3) + 0] + gr[(v * 3) + 0]) / 2, (gl[(v * 3) + 1] + gr[(v * 3) + 1]) / 2,
+                           (gl[(v * 3) + 2] + gr[(v * 3) + 2]) / 2);
+            }
+        }
+        V3<S> cu = v3(g.corr[(0 * nf) + f], g.corr[(1 * nf) + f], g.corr[(2 * nf) + f]);
+        S cn = g.corr[(3 * nf) + f];
+        for (int v = 0; v < 5; v++) {  // correct, derivative.c:81-89
+            S proj = dot(gm[v], cu);
+            S adjust = proj - ((pr[v] - pl[v]) / cn);
+            V3<S> a3 = mul(adjust, cu);
+            gm[v].x -= a3.x;
+            gm[v].y -= a3.y;
+            gm[v].z -= a3.z;
+        }
+        V3<S> nrm = v3(g.basis[(0 * nf) + f], g.basis[(1 * nf) + f], g.basis[(2 * nf) + f]);
+        S fl[5];
+        viscous_central<S>(fl, pm, gm, gamma, mu, prl, nrm);
+        for (int v = 0; v < 5; v++) fv[v] = fl[v] * area;
+    }
+    if (FLUX != FLUX_NONE) {
+        S ql[5], qr[5];
+        if (ORDER == 2) {  // reconstruct with the limited gradient G*psi (derivative.c:158-164)
+            V3<S> ol = v3(g.offl[(0 * nf) + f], g.offl[(1 * nf) + f], g.offl[(2 * nf) + f]);
+            for (int v = 0; v < 5; v++) {
+                S ps = gl[15 + v];
+                V3<S> gv = v3(gl[(v * 3) + 0] * ps, gl[(v * 3) + 1] * ps, gl[(v * 3) + 2] * ps);
+                ql[v] = pl[v] + dot(gv, ol);
+            }
+            if (bnd) {  // raw ghost state (derivative.c:203-211)
+                for (int v = 0; v < 5; v++) qr[v] = pr[v];
+            }
+            else {
+                V3<S> orr = v3(g.offr[(0 * nf) + f], g.offr[(1 * nf) + f], g.offr[(2 * nf) + f]);
+                for (int v = 0; v < 5; v++) {
+                    S ps = gr[15 + v];
+                    V3<S> gv = v3(gr[(v * 3) + 0] * ps, gr[(v * 3) + 1] * ps, gr[(v * 3) + 2] * ps);
+                    qr[v] = pr[v] + dot(gv, orr);
+                }
+            }
+        }
+        else {
+            for (int v = 0; v < 5; v++) {
+                ql[v] = pl[v];
+                qr[v] = pr[v];
+            }
+        }
+        S b9[9];
+        for (int i = 0; i < 9; i++) b9[i] = g.basis[(i * nf) + f];
+        S flx[5];
+        convective<FLUX, S>(flx, ql, qr, gamma, m3(b9), err);
+        for (int v = 0; v < 5; v++) fc[v] = flx[v] * area;
+    }
+}
+
+// One thread per face: F[f] = {Fv*A (5, viscous), Fc*A (5, convective)}.
+template <typename S, int FLUX, int VISC, int ORDER>
+__global__ __launch_bounds__(BLOCK) void k_face(Topo t, Geom<S> g, Phys ph, const S *__restrict__ P,
+                                                const S *__restrict__ GP, S *__restrict__ F,
+                                                int *error)
+{
+    constexpr int NF = (VISC && FLUX != FLUX_NONE) ? 10 : 5;
+    const int f = (xcd_block() * BLOCK) + threadIdx.x;
+    if (f >= t.nf) {
+        return;
+    }
+    S fv[5], fc[5];
+    int err = 0;
+    face_flux<S, FLUX, VISC, ORDER>(t, g, ph, P, GP, f, fv, fc, &err);
+    S *out = &F[(long)f * NF];
+    if (VISC) {
+        for (int v = 0; v < 5; v++) out[v] = fv[v];
+    }
+    if (FLUX != FLUX_NONE) {
+        for (int v = 0; v < 5; v++) out[(VISC ? 5 : 0) + v] = fc[v];
+    }
+    if (FLUX == FLUX_GODUNOV && err) {
+        atomicMax(error, err);
+    }
+}
+
+// ---------------------------------------------------------------- accumulate + update
 struct StageArgs {
     double *R;            // MODE 0: derivative out [5][ni]
     const double *U0;     // MODE 1/2
@@ -241,170 +371,22 @@ struct StageArgs {
     int keep_u;           // MODE 2: next U0 = U (mixed) instead of prim2cons(P_new) (FP64)
 };
 
-template <typename S>
-__device__ inline void load5(S *dst, const S *__restrict__ src, int stride, int idx)
+// Stage epilogue of inner cell c (active) given its accumulated face sums: finalize
+// R = -R/V (derivative.c:253-258); MODE 0 writes R; MODE 1 writes P_new = cons2prim(U0 +
+// (R alpha) step) (advance.c:103-108); MODE 2 also produces the next step's U0, the dt minimum
+// and the residual partial sums of this workgroup (block-wide: all threads must call it).
+template <typename S, int MODE>
+__device__ inline void stage_epilogue(const Topo &t, const Geom<S> &g, const Phys &ph, const Acc<S> &acc,
+                                      int c, bool active, int blk, S *__restrict__ Pout,
+                                      const StageArgs &a)
 {
-    for (int v = 0; v < 5; v++) dst[v] = src[(v * stride) + idx];
-}
-
-// MODE 0: write R = dU/dt; MODE 1: RK stage, write P_new; MODE 2: last RK stage (+ U0,
-// dt min, residual partials).
-template <typename S, int FLUX, int VISC, int ORDER, int MODE>
-__global__ __launch_bounds__(BLOCK) void k_flux(Topo t, Geom<S> g, Phys ph,
-                                                const S *__restrict__ P,
-                                                const S *__restrict__ G,
-                                                const S *__restrict__ psi,
-                                                S *__restrict__ Pout, StageArgs a)
-{
-    const int blk = xcd_block();
-    const int c = (blk * BLOCK) + threadIdx.x;
-    const bool active = c < t.ni;
     const int ni = t.ni;
-    const int nc = t.nc;
-    const int nf = t.nf;
-    const S gamma = (S)ph.gamma;
     double r[5] = {0, 0, 0, 0, 0};
     double pnew[5] = {0, 0, 0, 0, 0};
     double unew[5] = {0, 0, 0, 0, 0};
     if (active) {
-        S pc[5];
-        load5(pc, P, nc, c);
-        Acc<S> acc;
-        acc.zero();
-        if (VISC) {
-            const S mu = (S)ph.mu;
-            const S pr = (S)ph.prandtl;
-            S gc[15];
-            for (int i = 0; i < 15; i++) gc[i] = G[(i * ni) + c];
-#pragma unroll
-            for (int k = 0; k < W; k++) {
-                int e = t.inc[(k * ni) + c];
-                if (e < 0) {
-                    continue;
-                }
-                int f = e >> 1;
-                int side = e & 1;
-                int n = t.nbr[(k * ni) + c];
-                bool bnd = (f >= t.nfi) && (f < t.nfb);
-                S pn[5];
-                load5(pn, P, nc, n);
-                S pm[5];
-                for (int v = 0; v < 5; v++) pm[v] = (pc[v] + pn[v]) / 2;
-                V3<S> gm[5];
-                if (bnd) {  // G[left] only (derivative.c:129-139); c is the left cell
-                    for (int v = 0; v < 5; v++) {
-                        gm[v] = v3(gc[(v * 3) + 0], gc[(v * 3) + 1], gc[(v * 3) + 2]);
-                    }
-                }
-                else {
-                    for (int v = 0; v < 5; v++) {
-                        gm[v].x = (gc[(v * 3) + 0] + G[(((v * 3) + 0) * ni) + n]) / 2;
-                        gm[v].y = (gc[(v * 3) + 1] + G[(((v * 3) + 1) * ni) + n]) / 2;
-                        gm[v].z = (gc[(v * 3) + 2] + G[(((v * 3) + 2) * ni) + n]) / 2;
-                    }
-                }
-                V3<S> cu = v3(g.corr[(0 * nf) + f], g.corr[(1 * nf) + f], g.corr[(2 * nf) + f]);
-                S cn = g.corr[(3 * nf) + f];
-                for (int v = 0; v < 5; v++) {  // correct, derivative.c:81-89
-                    S dlr = side ? (pc[v] - pn[v]) : (pn[v] - pc[v]);
-                    S proj = dot(gm[v], cu);
-                    S adjust = proj - (dlr / cn);
-                    V3<S> a3 = mul(adjust, cu);
-                    gm[v].x -= a3.x;
-                    gm[v].y -= a3.y;
-                    gm[v].z -= a3.z;
-                }
-                V3<S> nrm = v3(g.basis[(0 * nf) + f], g.basis[(1 * nf) + f], g.basis[(2 * nf) + f]);
-                S fl[5];
-                viscous_central<S>(fl, pm, gm, gamma, mu, pr, nrm);
-                S area = g.area[f];
-                for (int v = 0; v < 5; v++) {
-                    S x = fl[v] * area;
-                    if (side) {
-                        acc.add(v, x);
-                    }
-                    else {
-                        acc.sub(v, x);
-                    }
-                }
-            }
-        }
-        if (FLUX != FLUX_NONE) {
-            S gc[15];
-            if (ORDER == 2) {
-                S psc[5];
-                load5(psc, psi, ni, c);
-                for (int v = 0; v < 5; v++) {
-                    for (int d = 0; d < 3; d++) gc[(v * 3) + d] = G[(((v * 3) + d) * ni) + c] * psc[v];
-                }
-            }
-            int err = 0;
-#pragma unroll
-            for (int k = 0; k < W; k++) {
-                int e = t.inc[(k * ni) + c];
-                if (e < 0) {
-                    continue;
-                }
-                int f = e >> 1;
-                int side = e & 1;
-                int n = t.nbr[(k * ni) + c];
-                bool bnd = (f >= t.nfi) && (f < t.nfb);
-                S pn[5];
-                load5(pn, P, nc, n);
-                S qc[5], qn[5];
-                if (ORDER == 2) {
-                    const S *offc = side ? g.offr : g.offl;
-                    const S *offn = side ? g.offl : g.offr;
-                    V3<S> oc = v3(offc[(0 * nf) + f], offc[(1 * nf) + f], offc[(2 * nf) + f]);
-                    for (int v = 0; v < 5; v++) {
-                        qc[v] = pc[v] + dot(v3(gc[(v * 3) + 0], gc[(v * 3) + 1], gc[(v * 3) + 2]), oc);
-                    }
-                    if (bnd) {  // raw ghost state (derivative.c:203-211)
-                        for (int v = 0; v < 5; v++) qn[v] = pn[v];
-                    }
-                    else {
-                        V3<S> on = v3(offn[(0 * nf) + f], offn[(1 * nf) + f], offn[(2 * nf) + f]);
-                        for (int v = 0; v < 5; v++) {
-                            S ps = psi[(v * ni) + n];
-                            V3<S> gn = v3(G[(((v * 3) + 0) * ni) + n] * ps,
-                                          G[(((v * 3) + 1) * ni) + n] * ps,
-                                          G[(((v * 3) + 2) * ni) + n] * ps);
-                            qn[v] = pn[v] + dot(gn, on);
-                        }
-                    }
-                }
-                else {
-                    for (int v = 0; v < 5; v++) {
-                        qc[v] = pc[v];
-                        qn[v] = pn[v];
-                    }
-                }
-                S b9[9];
-                for (int i = 0; i < 9; i++) b9[i] = g.basis[(i * nf) + f];
-                S fl[5];
-                if (side) {
-                    convective<FLUX, S>(fl, qn, qc, gamma, m3(b9), &err);
-                }
-                else {
-                    convective<FLUX, S>(fl, qc, qn, gamma, m3(b9), &err);
-                }
-                S area = g.area[f];
-                for (int v = 0; v < 5; v++) {
-                    S x = fl[v] * area;
-                    if (side) {
-                        acc.sub(v, x);
-                    }
-                    else {
-                        acc.add(v, x);
-                    }
-                }
-            }
-            if (FLUX == FLUX_GODUNOV && err) {
-                atomicMax(&a.scalars->error, err);
-            }
-        }
         double vol = (double)g.vol[c];
-        for (int v = 0; v < 5; v++) r[v] = -acc.get(v) / vol;  // derivative.c:253-258
+        for (int v = 0; v < 5; v++) r[v] = -acc.get(v) / vol;
         if (MODE == 0) {
             for (int v = 0; v < 5; v++) a.R[(v * ni) + c] = r[v];
         }
@@ -412,7 +394,7 @@ __global__ __launch_bounds__(BLOCK) void k_flux(Topo t, Geom<S> g, Phys ph,
             double step = a.scalars->step;
             for (int v = 0; v < 5; v++) unew[v] = a.U0[(v * ni) + c] + ((r[v] * a.alpha) * step);
             cons2prim<double>(pnew, unew, ph.gamma);
-            for (int v = 0; v < 5; v++) Pout[(v * nc) + c] = (S)pnew[v];
+            for (int v = 0; v < 5; v++) Pout[((long)c * 5) + v] = (S)pnew[v];
         }
     }
     if (MODE == 2) {
@@ -464,6 +446,127 @@ __global__ __launch_bounds__(BLOCK) void k_flux(Topo t, Geom<S> g, Phys ph,
     }
 }
 
+// Sum of the face contributions of inner cell c in reference face order: viscous
+// (R[left] -= Fv*A, R[right] += Fv*A, derivative.c:123-126) then convective (R[left] += Fc*A,
+// R[right] -= Fc*A, derivative.c:197-200).  `fetch(e, part)` returns the 5 values of the
+// face of incidence e (part 0: viscous, 1: convective).
+template <typename S, int VISC, int CONV, typename Fetch>
+__device__ inline void accumulate(const Topo &t, const int *__restrict__ slots, int c, Acc<S> &acc,
+                                  Fetch fetch)
+{
+    const int ni = t.ni;
+    acc.zero();
+    if (VISC) {
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            int e = slots[(k * ni) + c];
+            if (e < 0) {
+                continue;
+            }
+            S x[5];
+            fetch(e, 0, x);
+            for (int v = 0; v < 5; v++) {
+                if (e & 1) {
+                    acc.add(v, x[v]);
+                }
+                else {
+                    acc.sub(v, x[v]);
+                }
+            }
+        }
+    }
+    if (CONV) {
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            int e = slots[(k * ni) + c];
+            if (e < 0) {
+                continue;
+            }
+            S x[5];
+            fetch(e, 1, x);
+            for (int v = 0; v < 5; v++) {
+                if (e & 1) {
+                    acc.sub(v, x[v]);
+                }
+                else {
+                    acc.add(v, x[v]);
+                }
+            }
+        }
+    }
+}
+
+// Accumulate from the face flux array F written by k_face.
+template <typename S, int VISC, int CONV, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_accum(Topo t, Geom<S> g, Phys ph, const S *__restrict__ F,
+                                                 S *__restrict__ Pout, StageArgs a)
+{
+    constexpr int NF = (VISC && CONV) ? 10 : 5;
+    const int blk = xcd_block();
+    const int c = (blk * BLOCK) + threadIdx.x;
+    const bool active = c < t.ni;
+    Acc<S> acc;
+    acc.zero();
+    if (active) {
+        accumulate<S, VISC, CONV>(t, t.inc, c, acc, [&](int e, int part, S *x) {
+            const S *rec = &F[((long)(e >> 2) * NF) + ((part && VISC) ? 5 : 0)];
+            for (int v = 0; v < 5; v++) x[v] = rec[v];
+        });
+    }
+    stage_epilogue<S, MODE>(t, g, ph, acc, c, active, blk, Pout, a);
+}
+
+// Per-block face lists of the fused kernel: the workgroup of device cells
+// [b*BLOCK, (b+1)*BLOCK) computes the fluxes of every face touching them into LDS, then each
+// cell sums its faces from there.  slot = position in the block list << 1 | side.
+struct BlockTopo {
+    const int *__restrict__ off;   // [num_blocks + 1]
+    const int *__restrict__ face;  // device face index per list entry
+    const int *__restrict__ slot;  // [W][ni]
+    int max_faces;                 // longest list (LDS rows)
+};
+
+template <typename S, int FLUX, int VISC, int ORDER, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_flux_block(Topo t, BlockTopo bt, Geom<S> g, Phys ph,
+                                                      const S *__restrict__ P,
+                                                      const S *__restrict__ GP,
+                                                      S *__restrict__ Pout, StageArgs a)
+{
+    constexpr int CONV = FLUX != FLUX_NONE;
+    extern __shared__ unsigned char lds_raw[];
+    S *lds = (S *)lds_raw;  // [10][max_faces]: viscous 0..4, convective 5..9
+    const int mf = bt.max_faces;
+    const int blk = xcd_block();
+    const int beg = bt.off[blk];
+    const int num = bt.off[blk + 1] - beg;
+    int err = 0;
+    for (int j = threadIdx.x; j < num; j += BLOCK) {
+        S fv[5], fc[5];
+        face_flux<S, FLUX, VISC, ORDER>(t, g, ph, P, GP, bt.face[beg + j], fv, fc, &err);
+        if (VISC) {
+            for (int v = 0; v < 5; v++) lds[(v * mf) + j] = fv[v];
+        }
+        if (CONV) {
+            for (int v = 0; v < 5; v++) lds[((5 + v) * mf) + j] = fc[v];
+        }
+    }
+    if (FLUX == FLUX_GODUNOV && err) {
+        atomicMax(&a.scalars->error, err);
+    }
+    __syncthreads();
+    const int c = (blk * BLOCK) + threadIdx.x;
+    const bool active = c < t.ni;
+    Acc<S> acc;
+    acc.zero();
+    if (active) {
+        accumulate<S, VISC, CONV>(t, bt.slot, c, acc, [&](int e, int part, S *x) {
+            const int j = e >> 1;
+            for (int v = 0; v < 5; v++) x[v] = lds[(((part ? 5 : 0) + v) * mf) + j];
+        });
+    }
+    stage_epilogue<S, MODE>(t, g, ph, acc, c, active, blk, Pout, a);
+}
+
 // ---------------------------------------------------------------- time step + U0
 template <typename S>
 __global__ __launch_bounds__(BLOCK) void k_timestep_u0(Topo t, Geom<S> g, Phys ph,
@@ -474,7 +577,7 @@ __global__ __launch_bounds__(BLOCK) void k_timestep_u0(Topo t, Geom<S> g, Phys p
     double dt = __longlong_as_double(0x7ff0000000000000LL);
     if (c < t.ni) {
         double p[5];
-        for (int v = 0; v < 5; v++) p[v] = (double)P[(v * t.nc) + c];
+        for (int v = 0; v < 5; v++) p[v] = (double)P[((long)c * 5) + v];
         double vol = (double)g.vol[c];
         V3<double> pj = v3((double)g.proj[c], (double)g.proj[t.ni + c], (double)g.proj[(2 * t.ni) + c]);
         dt = ph.navierstokes ? ns_dt<double>(p, ph.gamma, ph.mu, ph.prandtl, vol, pj)
@@ -492,22 +595,38 @@ __global__ __launch_bounds__(BLOCK) void k_timestep_u0(Topo t, Geom<S> g, Phys p
 }
 
 // ---------------------------------------------------------------- layout conversion
-template <typename S>
-__global__ void k_aos_to_soa(const double *__restrict__ in, S *__restrict__ out, int num,
-                             int stride, int width, int out_stride)
+// host rows (reference numbering, out/in_stride doubles each) <-> device rows (device
+// numbering): device row i is host row perm[i] (identity when perm is null).  SOA selects a
+// [width][num] device array instead of [num] records of `stride` values.
+template <typename S, int SOA>
+__global__ void k_to_device(const double *__restrict__ in, S *__restrict__ out, int num,
+                            int in_stride, int width, int out_stride, const int *__restrict__ perm)
 {
     int i = (blockIdx.x * BLOCK) + threadIdx.x;
     if (i >= num) return;
-    for (int v = 0; v < width; v++) out[(v * out_stride) + i] = (S)in[((long)i * stride) + v];
+    long src = perm ? perm[i] : i;
+    for (int v = 0; v < width; v++) {
+        S x = (S)in[(src * in_stride) + v];
+        if (SOA) {
+            out[((long)v * num) + i] = x;
+        }
+        else {
+            out[((long)i * out_stride) + v] = x;
+        }
+    }
 }
 
-template <typename S>
-__global__ void k_soa_to_aos(const S *__restrict__ in, double *__restrict__ out, int num,
-                             int stride, int width, int in_stride)
+template <typename S, int SOA>
+__global__ void k_to_host(const S *__restrict__ in, double *__restrict__ out, int num,
+                          int in_stride, int width, int out_stride, const int *__restrict__ perm)
 {
     int i = (blockIdx.x * BLOCK) + threadIdx.x;
     if (i >= num) return;
-    for (int v = 0; v < width; v++) out[((long)i * stride) + v] = (double)in[(v * in_stride) + i];
+    long dst = perm ? perm[i] : i;
+    for (int v = 0; v < width; v++) {
+        S x = SOA ? in[((long)v * num) + i] : in[((long)i * in_stride) + v];
+        out[(dst * out_stride) + v] = (double)x;
+    }
 }
 
 }  // namespace fv

@@ -42,6 +42,19 @@ RESIDUAL_CASES = [
         ("left", "supersonic outflow", None), ("right", "supersonic outflow", None)]), "random"),
 ]
 
+# meshes of many 256-cell device blocks (full and partial blocks, block faces shared between
+# blocks, walls cutting through blocks)
+LARGE_CASES = [
+    ("ns_tgv_24", Case((24, 24, 24), lo=(0, 0, 0), hi=(TWO_PI,) * 3, kind="navierstokes",
+                       mu=1 / 1600), "tgv"),
+    ("euler_walls_17", Case((17, 13, 11), periodic=(0, 0, 0), flux="hllc", bcs=[
+        ("left", "supersonic inflow", (1.0, 2.0, 0.1, 0.0, 0.7)),
+        ("right", "supersonic outflow", None),
+        ("bottom", "symmetry", None), ("top", "slipwall", None),
+        ("back", "subsonic inflow", (1.1, 0.1, 0.0, 0.2, 1.0)),
+        ("front", "subsonic outflow", (1.0, 0.0, 0.0, 0.0, 0.9))]), "random"),
+]
+
 
 def initial_state(case, arrays, kind, seed=7):
     nc, ni = arrays["num_cells"], arrays["num_inner"]
