@@ -17,6 +17,50 @@ OMesh *oracle_mesh_create(const double *min, const double *max, const int *n, co
     return omesh_create(min, max, n, periodic);
 }
 
+static void *dup(const void *src, size_t bytes)
+{
+    void *dst = o_calloc(1, bytes ? bytes : 1);
+    if (src && bytes) {
+        memcpy(dst, src, bytes);
+    }
+    return dst;
+}
+
+/* Copy of a created, not yet generated mesh of the same layout (nodes, cell->node graph,
+ * entities, neighbour blocks), e.g. a tetrahedral mesh built by another library, so the
+ * oracle can generate and own it. */
+OMesh *oracle_mesh_copy_created(const OMesh *src)
+{
+    OMesh *m = o_calloc(1, sizeof(OMesh));
+    int nn = src->nodes.num, nc = src->cells.num, ne = src->entities.num;
+    int nb = src->neighbors.num;
+    m->nodes.num = nn;
+    m->nodes.num_inner = src->nodes.num_inner;
+    m->nodes.global = dup(src->nodes.global, nn * sizeof(long));
+    m->nodes.coord = dup(src->nodes.coord, nn * sizeof(OVec));
+    m->cells.num = nc;
+    m->cells.num_inner = src->cells.num_inner;
+    m->cells.off_boundary = src->cells.off_boundary;
+    m->cells.off_periodic = src->cells.off_periodic;
+    m->cells.node.off = dup(src->cells.node.off, (nc + 1) * sizeof(int));
+    m->cells.node.idx = dup(src->cells.node.idx, src->cells.node.off[nc] * sizeof(int));
+    m->entities.num = ne;
+    m->entities.num_inner = src->entities.num_inner;
+    m->entities.off_boundary = src->entities.off_boundary;
+    m->entities.name = dup(src->entities.name, ne * sizeof(OString));
+    m->entities.cell_off = dup(src->entities.cell_off, (ne + 1) * sizeof(int));
+    m->entities.rotation = dup(src->entities.rotation, ne * sizeof(OMat));
+    m->entities.translation = dup(src->entities.translation, ne * sizeof(OVec));
+    m->neighbors.num = nb;
+    m->neighbors.tag = dup(src->neighbors.tag, (nb + 1) * sizeof(int[2]));
+    m->neighbors.rank = dup(src->neighbors.rank, (nb + 1) * sizeof(int));
+    m->neighbors.recv_off = dup(src->neighbors.recv_off, (nb + 1) * sizeof(int));
+    m->neighbors.send.off = dup(src->neighbors.send.off, (nb + 1) * sizeof(int));
+    int nsend = src->neighbors.send.off ? src->neighbors.send.off[nb] : 0;
+    m->neighbors.send.idx = dup(src->neighbors.send.idx, (nsend + 1) * sizeof(int));
+    return m;
+}
+
 void oracle_mesh_generate(OMesh *mesh) { omesh_generate(mesh); }
 void oracle_mesh_destroy(OMesh *mesh) { omesh_destroy(mesh); }
 int oracle_periodic_source(const OMesh *mesh, int ghost) { return omesh_periodic_source(mesh, ghost); }

@@ -138,7 +138,6 @@ void o_fatal(const char *fmt, ...) __attribute__((noreturn, format(printf, 1, 2)
 /* ---- mesh (src/mesh/create.c, generate.c, reorder.c) ---- */
 OMesh *omesh_create(const double *min_coord, const double *max_coord, const int *num_cells,
                     const int *periodic);
-OMesh *omesh_create_tets(int n, double jitter, unsigned long long seed);
 void omesh_generate(OMesh *mesh);
 void omesh_destroy(OMesh *mesh);
 int omesh_periodic_source(const OMesh *mesh, int ghost);

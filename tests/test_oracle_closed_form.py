@@ -1,0 +1,502 @@
+"""Closed-form pins of the CPU oracle (no reference build needed).
+
+Each check is a property the mathematics fixes, chosen so that a plausible slip in the
+restatement (a dropped term, a wrong sign or index, a transposed basis) fails it:
+flux consistency, supersonic upwinding, rotational and mirror invariance, Toro's exact
+Riemann star states, boundary-condition invariants, viscous stress / heat-flux laws,
+LSQ gradient exactness on linear fields and the reference's own convergence test
+(test/2_equations/gradient.c:94-101), limiter bounds, free-stream preservation, discrete
+conservation, the uniform-hex time step and the dual graph by brute force."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from cases import RESIDUAL_CASES, initial_state
+from oracle_lib import Case, arr3, dp, oracle, oracle_eqns, oracle_mesh
+from teal_amd import _abi
+
+FLUXES = ["godunov", "roe", "hll", "hllc", "hlle", "ausmd", "ausmdv", "lxf", "central"]
+UPWIND = ["godunov", "roe", "hll", "hllc", "hlle", "ausmd", "ausmdv"]
+GAMMA = 1.4
+
+
+def prop(gamma=GAMMA, mu=0.0, pr=0.71):
+    return np.array([gamma, mu, pr, 0, 0, 0, 0, 0], dtype=np.float64)
+
+
+def rotation(rng):
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    q = q * np.sign(np.diag(r))
+    if np.linalg.det(q) < 0:
+        q[2] *= -1
+    return np.ascontiguousarray(q)
+
+
+def flux(name, left, right, basis, gamma=GAMMA):
+    out = np.zeros(5)
+    oracle().oracle_flux(name.encode(), dp(out), dp(np.ascontiguousarray(left, dtype=float)),
+                         dp(np.ascontiguousarray(right, dtype=float)), dp(prop(gamma)),
+                         dp(np.ascontiguousarray(basis, dtype=float)))
+    return out
+
+
+def physical(p, n, gamma=GAMMA):
+    """Euler flux F(U).n of the primitive state p = (rho, u, v, w, p)."""
+    rho, v, pr = p[0], np.asarray(p[1:4]), p[4]
+    un = v @ n
+    energy = pr / (gamma - 1) + 0.5 * rho * (v @ v)
+    return np.concatenate([[rho * un], rho * v * un + pr * n, [(energy + pr) * un]])
+
+
+def random_prim(rng, speed=0.5):
+    return np.concatenate([[rng.uniform(0.5, 1.5)], rng.uniform(-speed, speed, 3),
+                           [rng.uniform(0.5, 1.5)]])
+
+
+# ---------------------------------------------------------------- convective fluxes
+@pytest.mark.parametrize("name", FLUXES)
+def test_flux_consistency(name):
+    """F(u, u) = F(u).n on random states and random orthonormal bases."""
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        p = random_prim(rng)
+        b = rotation(rng)
+        exact = physical(p, b[0])
+        np.testing.assert_allclose(flux(name, p, p, b), exact, rtol=0,
+                                   atol=1e-13 * np.abs(exact).max())
+
+
+@pytest.mark.parametrize("name", UPWIND)
+def test_flux_supersonic_upwind(name):
+    """Both states supersonic along the normal: the flux is the left state's flux."""
+    rng = np.random.default_rng(2)
+    for _ in range(20):
+        b = rotation(rng)
+        pl, pr = random_prim(rng, 0.2), random_prim(rng, 0.2)
+        pl[1:4] += 4.0 * b[0]
+        pr[1:4] += 4.0 * b[0]
+        exact = physical(pl, b[0])
+        np.testing.assert_allclose(flux(name, pl, pr, b), exact, rtol=0,
+                                   atol=1e-12 * np.abs(exact).max())
+
+
+@pytest.mark.parametrize("name", FLUXES)
+def test_flux_rotation_invariance(name):
+    """Rotating both states and the face basis rotates the momentum flux the same way."""
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        pl, pr, b, q = random_prim(rng), random_prim(rng), rotation(rng), rotation(rng)
+        f = flux(name, pl, pr, b)
+        ql, qr = pl.copy(), pr.copy()
+        ql[1:4] = q @ pl[1:4]
+        qr[1:4] = q @ pr[1:4]
+        g = flux(name, ql, qr, b @ q.T)
+        expect = f.copy()
+        expect[1:4] = q @ f[1:4]
+        np.testing.assert_allclose(g, expect, rtol=0, atol=1e-12 * np.abs(f).max())
+
+
+@pytest.mark.parametrize("name", FLUXES)
+def test_flux_mirror_symmetry(name):
+    """Swapping the sides and reversing the normal reverses the flux."""
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        pl, pr, b = random_prim(rng), random_prim(rng), rotation(rng)
+        flip = np.array([-b[0], -b[1], b[2]])
+        np.testing.assert_allclose(flux(name, pr, pl, flip), -flux(name, pl, pr, b), rtol=0,
+                                   atol=1e-12 * np.abs(flux(name, pl, pr, b)).max())
+
+
+# Toro, Riemann Solvers and Numerical Methods for Fluid Dynamics, Table 4.3 (the cases of
+# run/riemann/toro1-5.c): left, right, p*, u*, rho*L, rho*R as printed (each value is held to
+# one unit of its last printed digit: the table truncates), sampling offset from u*
+TORO = [
+    ((1.0, 0.0, 1.0), (0.125, 0.0, 0.1), "0.30313", "0.92745", "0.42632", "0.26557", 0.05),
+    ((1.0, -2.0, 0.4), (1.0, 2.0, 0.4), "0.00189", "0.00000", "0.02185", "0.02185", 0.05),
+    ((1.0, 0.0, 1000.0), (1.0, 0.0, 0.01), "460.894", "19.5975", "0.57506", "5.99924", 1.0),
+    ((1.0, 0.0, 0.01), (1.0, 0.0, 100.0), "46.0950", "-6.19633", "5.99242", "0.57511", 0.5),
+    ((5.99924, 19.5975, 460.894), (5.99242, -6.19633, 46.0950), "1691.64", "8.68975",
+     "14.2823", "31.0426", 1.0),
+]
+
+
+def printed(value, actual, rtol=0.0):
+    """|actual - value| within one unit of the last printed digit of `value` (or rtol)."""
+    digits = len(value.split(".")[1]) if "." in value else 0
+    tol = max(10.0 ** -digits * (1 + 1e-9), rtol * abs(float(value)))
+    assert abs(actual - float(value)) <= tol, (actual, value)
+
+
+@pytest.mark.parametrize("case", TORO, ids=[f"toro{i + 1}" for i in range(5)])
+def test_exact_riemann_star_states(case):
+    (rl, ul, pl), (rr, ur, pr), ps, us, rsl, rsr, d = case
+    left = np.array([rl, ul, 0, 0, pl], dtype=float)
+    right = np.array([rr, ur, 0, 0, pr], dtype=float)
+    # test 5 starts from the six-digit star states of tests 3 and 4, so its star state is
+    # only known to about that relative precision
+    rtol = 1e-5 if case is TORO[4] else 0.0
+    for loc, rho in ((float(us) - d, rsl), (float(us) + d, rsr)):
+        out = np.zeros(5)
+        oracle().oracle_riemann(dp(left), dp(right), GAMMA, loc, dp(out))
+        printed(ps, out[4], rtol)
+        printed(us, out[1], rtol)
+        printed(rho, out[0], rtol)
+
+
+def test_godunov_flux_is_the_sampled_exact_state():
+    """Godunov flux along x = physical flux of the exact solution at x/t = 0."""
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        pl, pr = random_prim(rng), random_prim(rng)
+        out = np.zeros(5)
+        oracle().oracle_riemann(dp(pl), dp(pr), GAMMA, 0.0, dp(out))
+        exact = physical(out, np.array([1.0, 0, 0]))
+        np.testing.assert_allclose(flux("godunov", pl, pr, np.eye(3)), exact, rtol=0,
+                                   atol=1e-12 * np.abs(exact).max())
+
+
+# ---------------------------------------------------------------- boundary conditions
+def bc(name, inner, basis, ctx=None):
+    out = np.zeros(5)
+    c = None if ctx is None else dp(np.ascontiguousarray(ctx, dtype=float))
+    oracle().oracle_bc(name.encode(), dp(out), dp(np.ascontiguousarray(inner, dtype=float)),
+                       dp(prop()), dp(np.ascontiguousarray(basis)), c)
+    return out
+
+
+def test_bc_invariants():
+    rng = np.random.default_rng(6)
+    for _ in range(20):
+        b = rotation(rng)
+        n = b[0]
+        p = random_prim(rng)
+        ctx = random_prim(rng)
+        c2 = GAMMA * p[4] / p[0]
+        # symmetry / slip wall: normal velocity reversed, tangential kept, rho and p kept
+        for name in ("symmetry", "slipwall"):
+            o = bc(name, p, b)
+            assert o[0] == p[0] and o[4] == p[4]
+            np.testing.assert_allclose(o[1:4] @ n, -(p[1:4] @ n), atol=1e-15)
+            np.testing.assert_allclose(b[1:] @ o[1:4], b[1:] @ p[1:4], atol=1e-15)
+        # no-slip adiabatic wall: mirrored velocity, rho and p kept
+        for name in ("adiabatic wall", "wall"):
+            o = bc(name, p, b)
+            np.testing.assert_array_equal(o, [p[0], -p[1], -p[2], -p[3], p[4]])
+        np.testing.assert_array_equal(bc("supersonic inflow", p, b, ctx), ctx)
+        np.testing.assert_array_equal(bc("supersonic outflow", p, b, ctx), p)
+        # subsonic outflow: prescribed pressure, isentropic density change, acoustic
+        # relation |du_n| = |dp| / (rho c), tangential velocity kept
+        o = bc("subsonic outflow", p, b, ctx)
+        dpres = o[4] - p[4]
+        assert o[4] == ctx[4]
+        np.testing.assert_allclose(o[0] - p[0], dpres / c2, rtol=1e-12, atol=1e-15)
+        np.testing.assert_allclose(abs((o[1:4] - p[1:4]) @ n), abs(dpres) / (p[0] * np.sqrt(c2)),
+                                   rtol=1e-12, atol=1e-15)
+        np.testing.assert_allclose(b[1:] @ o[1:4], b[1:] @ p[1:4], atol=1e-15)
+        # subsonic inflow: tangential velocity and entropy-consistent density from the context
+        o = bc("subsonic inflow", p, b, ctx)
+        np.testing.assert_allclose(b[1:] @ o[1:4], b[1:] @ ctx[1:4], atol=1e-14)
+        np.testing.assert_allclose(o[0] - ctx[0], (o[4] - ctx[4]) / c2, rtol=1e-12, atol=1e-15)
+        # pressure outflow: subsonic -> context pressure at the inner temperature
+        o = bc("pressure outflow", p, b, ctx)
+        assert o[4] == ctx[4]
+        np.testing.assert_allclose(o[4] / o[0], p[4] / p[0], rtol=1e-14)
+        np.testing.assert_array_equal(o[1:4], p[1:4])
+        fast = p.copy()
+        fast[1:4] += 3.0 * n
+        np.testing.assert_allclose(bc("pressure outflow", fast, b, ctx), fast, rtol=1e-15)
+
+
+# ---------------------------------------------------------------- viscous flux
+def viscous(p, g, basis, mu, pr=0.71):
+    out = np.zeros(5)
+    oracle().oracle_viscous(dp(out), dp(np.ascontiguousarray(p, dtype=float)),
+                            dp(np.ascontiguousarray(g, dtype=float)), dp(prop(GAMMA, mu, pr)),
+                            dp(np.ascontiguousarray(basis, dtype=float)))
+    return out
+
+
+def test_viscous_laws():
+    mu, pr = 0.05, 0.71
+    p = np.array([1.3, 0.4, -0.2, 0.1, 0.9])
+    e = np.eye(3)
+    b_y = np.array([e[1], e[2], e[0]])
+    # simple shear u = s*y across a y-face: tau_xy = mu s, work = tau_xy u
+    s = 0.7
+    g = np.zeros((5, 3))
+    g[1] = [0, s, 0]
+    f = viscous(p, g, b_y, mu)
+    np.testing.assert_allclose(f, [0, mu * s, 0, 0, mu * s * p[1]], rtol=1e-14, atol=1e-17)
+    # uniaxial strain u = a*x across an x-face (Stokes hypothesis): tau_xx = 4/3 mu a
+    a = 0.3
+    g = np.zeros((5, 3))
+    g[1] = [a, 0, 0]
+    f = viscous(p, g, e, mu)
+    np.testing.assert_allclose(f, [0, 4 / 3 * mu * a, 0, 0, 4 / 3 * mu * a * p[1]], rtol=1e-14)
+    # Fourier heat flux at rest, constant density: k dT/dn = mu gamma / ((gamma-1) Pr rho) dp/dn
+    q = np.array([1.0, 0, 0, 0, 2.0])
+    g = np.zeros((5, 3))
+    g[4] = [0.5, 0, 0]
+    f = viscous(q, g, e, mu, pr)
+    np.testing.assert_allclose(f, [0, 0, 0, 0, mu * GAMMA / ((GAMMA - 1) * pr) * 0.5],
+                               rtol=1e-14, atol=1e-18)
+    # no gradient, no viscous flux
+    np.testing.assert_array_equal(viscous(p, np.zeros((5, 3)), e, mu), np.zeros(5))
+
+
+# ---------------------------------------------------------------- conversions, time step
+def test_conversions():
+    u = np.zeros(5)
+    p = np.array([1.0, 1.0, 2.0, 3.0, 1.0])
+    oracle().oracle_prim2cons(dp(u), dp(p), dp(prop()))
+    np.testing.assert_array_equal(u, [1.0, 1.0, 2.0, 3.0, 2.5 + 7.0])
+    rng = np.random.default_rng(7)
+    for _ in range(50):
+        p = random_prim(rng)
+        u, back = np.zeros(5), np.zeros(5)
+        oracle().oracle_prim2cons(dp(u), dp(p), dp(prop()))
+        oracle().oracle_cons2prim(dp(back), dp(u), dp(prop()))
+        np.testing.assert_allclose(back, p, rtol=1e-14, atol=1e-15)
+
+
+@pytest.mark.parametrize("kind", ["euler", "navierstokes"])
+def test_timestep_uniform_hex(kind):
+    """Uniform hex (h = 1/n), uniform state: dt = V / (sum_d (|u_d| + c) h^2 + 4 nu |proj|^2/V)."""
+    n = 6
+    case = Case((n, n, n), kind=kind, mu=0.01 if kind == "navierstokes" else 0.0)
+    m = oracle_mesh(case)
+    e = oracle_eqns(case, m)
+    a = _abi.mesh_arrays(m)
+    p = np.array([1.2, 0.3, -0.4, 0.5, 0.8])
+    P = np.tile(p, (a["num_cells"], 1))
+    h = 1.0 / n
+    c = np.sqrt(GAMMA * p[4] / p[0])
+    lam = (np.abs(p[1:4]) + c).sum() * h * h
+    if kind == "navierstokes":
+        lv = max(4 * 0.01 / (3 * p[0]), GAMMA * 0.01 / (p[0] * 0.71))
+        lam += 4 * lv * 3 * h ** 4 / h ** 3
+    dt = oracle().oracle_timestep(e, dp(P))
+    np.testing.assert_allclose(dt, h ** 3 / lam, rtol=1e-13)
+
+
+# ---------------------------------------------------------------- gradient, limiter
+def interior_cells(a):
+    """Inner cells whose face neighbours are all inner cells."""
+    ni = a["num_inner"]
+    off, idx = a["cell_cell_off"], a["cell_cell_idx"]
+    return np.array([c for c in range(ni) if (idx[off[c]:off[c + 1]] < ni).all()])
+
+
+def generic_cells(a, cells):
+    """Cells none of whose neighbour centre differences has a zero component.  The reference
+    drops the weight terms of a component that is_close() to zero (src/mesh/generate.c:
+    678-690) although the Gram-Schmidt coefficient of that row is not zero unless the
+    stencil is orthogonal (as on hexes), so its LSQ gradient is exact for linear fields only
+    on such cells; the oracle keeps that behaviour."""
+    off, idx = a["cell_cell_off"], a["cell_cell_idx"]
+    x = a["center"]
+    keep = []
+    for c in cells:
+        d = x[idx[off[c]:off[c + 1]]] - x[c]
+        if (np.abs(d) > 1e-8).all():
+            keep.append(c)
+    return np.array(keep)
+
+
+def gradient(e, P, nc):
+    G = np.zeros((nc, 5, 3))
+    oracle().oracle_gradient(e, dp(P), dp(G))
+    return G
+
+
+def tet_case_mesh(n=5, jitter=0.25, seed=11):
+    import teal_amd
+
+    lib = oracle()
+    lib.oracle_mesh_copy_created.restype = C.c_void_p
+    lib.oracle_mesh_copy_created.argtypes = [C.c_void_p]
+    pm = teal_amd.lib().mesh_create_tets(n, jitter, seed)
+    om = lib.oracle_mesh_copy_created(pm)
+    teal_amd.lib().mesh_destroy(pm)
+    lib.oracle_mesh_generate(om)
+    return om
+
+
+@pytest.mark.parametrize("mesh", ["hex", "tets"])
+def test_gradient_exact_for_linear_fields(mesh):
+    if mesh == "hex":
+        case = Case((5, 4, 6), lo=(0, 0, 0), hi=(1.0, 2.0, 1.5), periodic=(0, 0, 0),
+                    bcs=[(s, "supersonic outflow", None)
+                         for s in ("left", "right", "bottom", "top", "back", "front")])
+        m = oracle_mesh(case)
+    else:
+        case = Case((5, 5, 5), periodic=(0, 0, 0),
+                    bcs=[(s, "supersonic outflow", None)
+                         for s in ("left", "right", "bottom", "top", "back", "front")])
+        m = tet_case_mesh()
+    e = oracle_eqns(case, m)
+    a = _abi.mesh_arrays(m)
+    nc = a["num_cells"]
+    x = a["center"][:nc]
+    slope = np.array([[0.5, -1.0, 2.0], [1.5, 0.25, -0.75], [-2.0, 1.0, 0.5],
+                      [0.1, 0.2, 0.3], [3.0, -0.5, 1.0]])
+    P = 1.0 + x @ slope.T
+    G = gradient(e, np.ascontiguousarray(P), nc)
+    cells = interior_cells(a)
+    if mesh == "tets":
+        cells = generic_cells(a, cells)
+    assert len(cells) > 10
+    np.testing.assert_allclose(G[cells], np.broadcast_to(slope, (len(cells), 5, 3)), rtol=0,
+                               atol=1e-11)
+
+
+def test_gradient_convergence_order():
+    """test/2_equations/gradient.c:27-104: periodic unit box, u = sin(2 pi x) cos(2 pi y)
+    cos(2 pi z), v = cos(2 pi x) sin(2 pi y) cos(2 pi z); observed order > 1.99."""
+    fac = 2 * np.pi
+    errors = []
+    for n in (16, 32, 64):
+        case = Case((n, n, n))
+        m = oracle_mesh(case)
+        e = oracle_eqns(case, m)
+        a = _abi.mesh_arrays(m)
+        nc, ni = a["num_cells"], a["num_inner"]
+        x, y, z = (a["center"][:nc, k] for k in range(3))
+        P = np.zeros((nc, 5))
+        P[:, 0] = 1.0
+        P[:, 1] = np.sin(fac * x) * np.cos(fac * y) * np.cos(fac * z)
+        P[:, 2] = np.cos(fac * x) * np.sin(fac * y) * np.cos(fac * z)
+        P[:, 4] = 1.0
+        G = gradient(e, P, nc)[:ni]
+        x, y, z = x[:ni], y[:ni], z[:ni]
+        exact = np.zeros((ni, 5, 3))
+        exact[:, 1, 0] = fac * np.cos(fac * x) * np.cos(fac * y) * np.cos(fac * z)
+        exact[:, 1, 1] = -fac * np.sin(fac * x) * np.sin(fac * y) * np.cos(fac * z)
+        exact[:, 1, 2] = -fac * np.sin(fac * x) * np.cos(fac * y) * np.sin(fac * z)
+        exact[:, 2, 0] = -fac * np.sin(fac * x) * np.sin(fac * y) * np.cos(fac * z)
+        exact[:, 2, 1] = fac * np.cos(fac * x) * np.cos(fac * y) * np.cos(fac * z)
+        exact[:, 2, 2] = -fac * np.cos(fac * x) * np.sin(fac * y) * np.sin(fac * z)
+        vol = a["volume"][:ni]
+        err = np.sqrt((vol[:, None] * ((G - exact) ** 2).sum(axis=2)).sum(axis=0) / vol.sum())
+        errors.append(err.max())
+        oracle().oracle_eqns_destroy(e)
+        oracle().oracle_mesh_destroy(m)
+    rates = np.log2(np.array(errors[:-1]) / np.array(errors[1:]))
+    assert (rates > 1.99).all(), rates
+
+
+@pytest.mark.parametrize("limiter", ["barth jespersen", "venkatakrishnan"])
+def test_limiter_bounds(limiter):
+    case = Case((6, 5, 4), limiter=limiter, coefficient=1.0)
+    m = oracle_mesh(case)
+    e = oracle_eqns(case, m)
+    a = _abi.mesh_arrays(m)
+    nc, ni = a["num_cells"], a["num_inner"]
+    P = initial_state(case, a, "random", seed=3)
+    G = gradient(e, P, nc)
+    L = G.copy()
+    oracle().oracle_limiter(e, dp(P), dp(L))
+    # L = G * psi per variable with 0 <= psi <= 1
+    with np.errstate(invalid="ignore", divide="ignore"):
+        psi = np.where(np.abs(G[:ni]) > 0, L[:ni] / G[:ni], np.nan)
+    psi = np.nanmean(psi, axis=2)
+    assert np.nanmin(psi) >= -1e-15 and np.nanmax(psi) <= 1 + 1e-15
+    if limiter == "barth jespersen":
+        # reconstructed face values stay inside the min/max of the cell and its neighbours
+        off, idx = a["cell_cell_off"], a["cell_cell_idx"]
+        for c in range(ni):
+            nb = np.concatenate([[c], idx[off[c]:off[c + 1]]])
+            lo, hi = P[nb].min(axis=0), P[nb].max(axis=0)
+            faces = reconstructed_faces(a, c, P, L)
+            assert (faces >= lo - 1e-12).all() and (faces <= hi + 1e-12).all()
+
+
+def reconstructed_faces(a, c, P, L):
+    """P_c + L_c . (x_f - x_c) over the faces of inner cell c."""
+    fl = a["cell_idx"]
+    sel = np.nonzero((fl[:, 0] == c) | (fl[:, 1] == c))[0]
+    d = a["face_center"][sel] - a["center"][c]
+    return P[c] + d @ L[c].T
+
+
+# ---------------------------------------------------------------- full residual
+@pytest.mark.parametrize("name", FLUXES)
+def test_free_stream_preservation(name):
+    """A uniform state on a jittered tetrahedral mesh has R = 0 up to round-off."""
+    case = Case((5, 5, 5), periodic=(0, 0, 0), kind="navierstokes", mu=0.01, flux=name,
+                bcs=[(s, "supersonic outflow", None)
+                     for s in ("left", "right", "bottom", "top", "back", "front")])
+    m = tet_case_mesh()
+    e = oracle_eqns(case, m)
+    a = _abi.mesh_arrays(m)
+    nc, ni = a["num_cells"], a["num_inner"]
+    p = np.array([1.1, 0.3, -0.2, 0.25, 0.9])
+    P = np.tile(p, (nc, 1))
+    R = np.zeros((ni, 5))
+    oracle().oracle_derivative(e, dp(P), dp(R))
+    h = 1.0 / 5
+    scale = np.abs(physical(p, np.array([1.0, 0, 0]))).max() / h
+    assert np.abs(R).max() <= 1e-11 * scale
+
+
+@pytest.mark.parametrize("name,case,kind", RESIDUAL_CASES[:3], ids=[c[0] for c in RESIDUAL_CASES[:3]])
+def test_discrete_conservation(name, case, kind):
+    """Periodic box: sum_i V_i R_i = 0 (every face flux enters two cells with opposite sign)."""
+    m = oracle_mesh(case)
+    e = oracle_eqns(case, m)
+    a = _abi.mesh_arrays(m)
+    ni = a["num_inner"]
+    P = initial_state(case, a, kind)
+    R = np.zeros((ni, 5))
+    oracle().oracle_derivative(e, dp(P), dp(R))
+    vol = a["volume"][:ni, None]
+    total = (vol * R).sum(axis=0)
+    assert (np.abs(total) <= 1e-12 * np.abs(vol * R).sum(axis=0) + 1e-14).all(), total
+
+
+# ---------------------------------------------------------------- dual graph
+def brute_dual(eptr, eind, ncommon):
+    """Elements sharing >= ncommon nodes (or all but one node of either element), listed in
+    first-encounter order: e's nodes in order, each node's elements by increasing id."""
+    ne = len(eptr) - 1
+    nodes = [list(eind[eptr[e]:eptr[e + 1]]) for e in range(ne)]
+    elems_of = {}
+    for e in range(ne):
+        for v in nodes[e]:
+            elems_of.setdefault(v, []).append(e)
+    out = []
+    for e in range(ne):
+        seen = []
+        for v in nodes[e]:
+            for f in elems_of[v]:
+                if f != e and f not in seen:
+                    seen.append(f)
+        keep = []
+        for f in seen:
+            ov = len(set(nodes[e]) & set(nodes[f]))
+            if ov >= ncommon or ov >= len(nodes[e]) - 1 or ov >= len(nodes[f]) - 1:
+                keep.append(f)
+        out.append(keep)
+    return out
+
+
+@pytest.mark.parametrize("shape", [(3, 2, 2), (2, 2, 1)])
+def test_dual_graph_brute_force(shape):
+    case = Case(shape, periodic=(0, 0, 0))
+    m = oracle().oracle_mesh_create(arr3(case.lo), arr3(case.hi), arr3(case.n, C.c_int),
+                                    arr3(case.periodic, C.c_int))
+    a = _abi.mesh_arrays(m)
+    off = a["cell_node_off"].astype(np.int64)
+    idx = a["cell_node_idx"].astype(np.int64)
+    ne = a["num_cells"]
+    nn = a["num_nodes"]
+    xadj = np.zeros(ne + 1, dtype=np.int64)
+    cap = 64 * ne
+    adj = np.zeros(cap, dtype=np.int64)
+    L = C.POINTER(C.c_long)
+    oracle().oracle_mesh_to_dual(ne, nn, off.ctypes.data_as(L), idx.ctypes.data_as(L), 3,
+                                 xadj.ctypes.data_as(L), adj.ctypes.data_as(L), cap)
+    got = [list(adj[xadj[e]:xadj[e + 1]]) for e in range(ne)]
+    assert got == brute_dual(off, idx, 3)
+    oracle().oracle_mesh_destroy(m)

@@ -1,0 +1,197 @@
+"""The oracle against the reference's own sources compiled into oracle/_ref (skipped where that
+build is absent): bit-for-bit on identical meshes and states, on inputs the golden fixtures
+do not hold (tetrahedral meshes, every lserk order/stage pair, the GMRES solve by itself,
+the gradient / limiter / time step / residual entry points).  The reference's mesh_validate
+also checks the oracle's mesh generation (the reference's own generate.c needs METIS)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from cases import LARGE_CASES, RESIDUAL_CASES, initial_state
+from oracle_lib import (Case, dp, have_ref, oracle, oracle_eqns, oracle_mesh, random_state,
+                        ref, ref_eqns)
+from teal_amd import _abi
+
+pytestmark = pytest.mark.skipif(not have_ref(), reason="oracle/_ref (reference build) absent")
+
+SIDES = ("left", "right", "bottom", "top", "back", "front")
+
+
+def tet_mesh(n=4, jitter=0.25, seed=5):
+    import teal_amd
+
+    lib = oracle()
+    lib.oracle_mesh_copy_created.restype = C.c_void_p
+    lib.oracle_mesh_copy_created.argtypes = [C.c_void_p]
+    pm = teal_amd.lib().mesh_create_tets(n, jitter, seed)
+    om = lib.oracle_mesh_copy_created(pm)
+    teal_amd.lib().mesh_destroy(pm)
+    lib.oracle_mesh_generate(om)
+    return om
+
+
+TET_CASES = [
+    ("tets_euler_hllc", Case((4, 4, 4), periodic=(0, 0, 0), flux="hllc", bcs=[
+        ("left", "supersonic inflow", (1.0, 1.5, 0.1, 0.0, 0.8)), ("right", "supersonic outflow", None),
+        ("bottom", "symmetry", None), ("top", "farfield", (1.0, 0.2, 0.0, 0.1, 1.0)),
+        ("back", "subsonic inflow", (1.1, 0.1, 0.0, 0.2, 1.0)),
+        ("front", "subsonic outflow", (1.0, 0.0, 0.0, 0.0, 0.9))])),
+    ("tets_ns_roe_vk", Case((4, 4, 4), periodic=(0, 0, 0), kind="navierstokes", mu=0.02,
+                            flux="roe", limiter="venkatakrishnan", coefficient=2.0,
+                            bcs=[(s, "wall", None) for s in SIDES])),
+    ("tets_euler_ausmdv_nolim", Case((4, 4, 4), periodic=(0, 0, 0), flux="ausmdv", limiter=None,
+                                     bcs=[(s, "pressure outflow", (1, 0, 0, 0, 0.9))
+                                          for s in SIDES])),
+]
+
+
+def both(case, mesh):
+    return oracle_eqns(case, mesh), ref_eqns(case, mesh)
+
+
+@pytest.mark.parametrize("name,case", TET_CASES, ids=[c[0] for c in TET_CASES])
+def test_tets_derivative_and_lserk(name, case):
+    m = tet_mesh()
+    ref().ref_mesh_validate(m)
+    eo, er = both(case, m)
+    a = _abi.mesh_arrays(m)
+    nc, ni = a["num_cells"], a["num_inner"]
+    P = random_state(nc, 9)
+    P[ni:] = 0
+    Po, Pr = P.copy(), P.copy()
+    Ro, Rr = np.zeros((ni, 5)), np.zeros((ni, 5))
+    oracle().oracle_derivative(eo, dp(Po), dp(Ro))
+    ref().ref_derivative(er, dp(Pr), dp(Rr))
+    np.testing.assert_array_equal(Ro, Rr)
+    np.testing.assert_array_equal(Po, Pr)
+    po = np.ctypeslib.as_array(oracle().oracle_eqns_primitive(eo), shape=(nc * 5,))
+    pr = np.ctypeslib.as_array(ref().ref_eqns_primitive(er), shape=(nc * 5,))
+    po[:] = P.ravel()
+    pr[:] = P.ravel()
+    to, tr = C.c_double(0), C.c_double(0)
+    ro, rr = (C.c_double * 5)(), (C.c_double * 5)()
+    for _ in range(3):
+        so = oracle().oracle_lserk(eo, C.byref(to), ro, 1e300, 0.6, 3, 3)
+        sr = ref().ref_lserk(er, C.byref(tr), rr, 1e300, 0.6, 3, 3)
+        assert so == sr and to.value == tr.value and ro[:] == rr[:]
+    np.testing.assert_array_equal(po, pr)
+
+
+def test_mesh_validate_accepts_oracle_meshes():
+    for n, per in (((7, 5, 3), (1, 0, 1)), ((9, 7, 1), (1, 0, 0)), ((12, 1, 1), (0, 0, 0)),
+                   ((16, 16, 16), (1, 1, 1))):
+        ref().ref_mesh_validate(oracle_mesh(Case(n, periodic=per)))
+
+
+def test_mesh_create_matches_reference():
+    """mesh_create of the reference's own mesh test (test/1_mesh/create.c:14-17)."""
+    from oracle_lib import arr3
+
+    args = (arr3((0.0, 0.0, 0.0)), arr3((3.0, 1.0, 1.0)), arr3((30, 10, 10), C.c_int),
+            arr3((1, 0, 1), C.c_int))
+    a = _abi.mesh_arrays(oracle().oracle_mesh_create(*args))
+    b = _abi.mesh_arrays(ref().ref_mesh_create(*args))
+    for k in a:
+        if isinstance(a[k], np.ndarray):
+            np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+        else:
+            assert a[k] == b[k], k
+
+
+@pytest.mark.parametrize("order,stages", [(o, s) for o in (1, 2, 3) for s in range(1, 7)])
+def test_lserk_every_scheme(order, stages):
+    name, case, kind = RESIDUAL_CASES[1]
+    m = oracle_mesh(case)
+    eo, er = both(case, m)
+    a = _abi.mesh_arrays(m)
+    nc = a["num_cells"]
+    P = initial_state(case, a, kind)
+    po = np.ctypeslib.as_array(oracle().oracle_eqns_primitive(eo), shape=(nc * 5,))
+    pr = np.ctypeslib.as_array(ref().ref_eqns_primitive(er), shape=(nc * 5,))
+    po[:] = P.ravel()
+    pr[:] = P.ravel()
+    to, tr = C.c_double(0), C.c_double(0)
+    ro, rr = (C.c_double * 5)(), (C.c_double * 5)()
+    for _ in range(2):
+        so = oracle().oracle_lserk(eo, C.byref(to), ro, 0.01, 0.4, order, stages)
+        sr = ref().ref_lserk(er, C.byref(tr), rr, 0.01, 0.4, order, stages)
+        assert so == sr and to.value == tr.value and ro[:] == rr[:]
+    np.testing.assert_array_equal(po, pr)
+
+
+@pytest.mark.parametrize("name,case,kind", RESIDUAL_CASES[:3] + RESIDUAL_CASES[12:14],
+                         ids=[c[0] for c in RESIDUAL_CASES[:3] + RESIDUAL_CASES[12:14]])
+def test_entry_points(name, case, kind):
+    """boundary, gradient, limiter, timestep and residual on their own."""
+    m = oracle_mesh(case)
+    eo, er = both(case, m)
+    a = _abi.mesh_arrays(m)
+    nc, ni = a["num_cells"], a["num_inner"]
+    P = initial_state(case, a, kind)
+    Po, Pr = P.copy(), P.copy()
+    oracle().oracle_boundary(eo, dp(Po))
+    ref().ref_boundary(er, dp(Pr))
+    np.testing.assert_array_equal(Po, Pr)
+    Go, Gr = np.zeros((nc, 15)), np.zeros((nc, 15))
+    oracle().oracle_gradient(eo, dp(Po), dp(Go))
+    ref().ref_gradient(er, dp(Pr), dp(Gr))
+    np.testing.assert_array_equal(Go, Gr)
+    if case.limiter:
+        oracle().oracle_limiter(eo, dp(Po), dp(Go))
+        ref().ref_limiter(er, dp(Pr), dp(Gr))
+        np.testing.assert_array_equal(Go, Gr)
+    assert oracle().oracle_timestep(eo, dp(Po)) == ref().ref_timestep(er, dp(Pr))
+    D = np.random.default_rng(3).normal(size=(ni, 5))
+    ro, rr = np.zeros(5), np.zeros(5)
+    oracle().oracle_residual(eo, dp(D), dp(ro))
+    ref().ref_residual(er, dp(D), dp(rr))
+    np.testing.assert_array_equal(ro, rr)
+
+
+@pytest.mark.parametrize("dim", [4, 16])
+def test_gmres_solve(dim):
+    """One matrix-free GMRES solve (src/simulation/gmres.c:53-140) on a NS state."""
+    name, case, kind = RESIDUAL_CASES[1]
+    m = oracle_mesh(case)
+    eo, er = both(case, m)
+    a = _abi.mesh_arrays(m)
+    nc, ni = a["num_cells"], a["num_inner"]
+    P = initial_state(case, a, kind)
+    prop = np.array([case.gamma, case.mu, case.prandtl, 0, 0, 0, 0, 0])
+    U = np.zeros((nc, 5))
+    for i in range(ni):
+        oracle().oracle_prim2cons(dp(U[i]), dp(P[i]), dp(prop))
+    Pd = P.copy()
+    D = np.zeros((nc, 5))
+    oracle().oracle_derivative(eo, dp(Pd), dp(D))
+    po = np.ctypeslib.as_array(oracle().oracle_eqns_primitive(eo), shape=(nc * 5,))
+    pr = np.ctypeslib.as_array(ref().ref_eqns_primitive(er), shape=(nc * 5,))
+    step = 0.05
+    rhs = -D * step
+    norm = float(np.sqrt((rhs[:ni] ** 2).sum()))
+    out = []
+    for lib, pre, e, p in ((oracle(), "oracle_", eo, po), (ref(), "ref_", er, pr)):
+        p[:] = P.ravel()
+        inc = np.zeros((nc, 5))
+        getattr(lib, pre + "gmres")(e, dp(U), dp(D), dp(rhs), dp(inc), step, norm, 1e-7, 1e-3,
+                                    dim)
+        out.append((inc.copy(), p.copy()))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    assert np.abs(out[0][0]).max() > 0
+
+
+@pytest.mark.parametrize("name,case,kind", LARGE_CASES, ids=[c[0] for c in LARGE_CASES])
+def test_large_derivative(name, case, kind):
+    m = oracle_mesh(case)
+    eo, er = both(case, m)
+    a = _abi.mesh_arrays(m)
+    ni = a["num_inner"]
+    P = initial_state(case, a, kind)
+    Po, Pr = P.copy(), P.copy()
+    Ro, Rr = np.zeros((ni, 5)), np.zeros((ni, 5))
+    oracle().oracle_derivative(eo, dp(Po), dp(Ro))
+    ref().ref_derivative(er, dp(Pr), dp(Rr))
+    np.testing.assert_array_equal(Ro, Rr)
+    np.testing.assert_array_equal(Po, Pr)
