@@ -92,7 +92,8 @@ struct TealGpu {
     DevData<float> d32;
     TealGpuConfig cfg{};
     int configured = 0;
-    int u0_valid = 0;
+    int u0_valid = 0;  // d_U0 holds the conserved state of the current step start
+    int dt_valid = 0;  // the scalars hold the dt minimum of that state
     int nblocks = 0;
     hipStream_t stream = nullptr;
     double *d_stage = nullptr;

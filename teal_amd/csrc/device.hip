@@ -119,6 +119,31 @@ __global__ void k_residual_partials(const double *__restrict__ R, const double *
     }
 }
 
+// U0 = prim2cons(P) in FP64 from host rows (reference numbering) for the inner cells in
+// device order: the step's U0 comes from the caller's FP64 state, also on the mixed path.
+__global__ void k_u0_from_host(const double *__restrict__ host, const int *__restrict__ perm,
+                               int ni, double gamma, double *__restrict__ U0)
+{
+    int d = (blockIdx.x * BLOCK) + threadIdx.x;
+    if (d >= ni) return;
+    const double *p = &host[(long)perm[d] * 5];
+    double u[5];
+    prim2cons<double>(u, p, gamma);
+    for (int v = 0; v < 5; v++) U0[((long)v * ni) + d] = u[v];
+}
+
+// host rows of the inner cells = cons2prim(U) in FP64 (the mixed path's FP64 solution)
+__global__ void k_prim_from_u(const double *__restrict__ U, const int *__restrict__ perm, int ni,
+                              double gamma, double *__restrict__ host)
+{
+    int d = (blockIdx.x * BLOCK) + threadIdx.x;
+    if (d >= ni) return;
+    double u[5], p[5];
+    for (int v = 0; v < 5; v++) u[v] = U[((long)v * ni) + d];
+    cons2prim<double>(p, u, gamma);
+    for (int v = 0; v < 5; v++) host[((long)perm[d] * 5) + v] = p[v];
+}
+
 // ------------------------------------------------------------------ mesh upload
 template <typename T>
 static int upload(T **dst, const std::vector<T> &src)
@@ -677,13 +702,21 @@ int teal_residual_pass(TealGpu *g, DevData<S> &d, int mode, double alpha, int la
 template int teal_residual_pass<double>(TealGpu *, DevData<double> &, int, double, int);
 template int teal_residual_pass<float>(TealGpu *, DevData<float> &, int, double, int);
 
-// host primitive rows (reference numbering) -> device P records
+// host primitive rows (reference numbering) -> device P records, and U0 = prim2cons(P) in FP64
+// (valid for the next explicit step; the time step still has to be computed)
 template <typename S>
 static int upload_primitive(TealGpu *g, DevData<S> &d, const double *primitive)
 {
     size_t bytes = sizeof(double) * 5 * (size_t)g->nc;
     HIPCALL(hipMemcpyAsync(g->d_stage, primitive, bytes, hipMemcpyHostToDevice, g->stream));
     Kern<S>::to_device(g->stream, g->d_stage, d.d_P, g->nc, 5, 5, 5, g->d_cell_perm, 0);
+    if (g->configured) {
+        hipLaunchKernelGGL(k_u0_from_host, dim3(g->nblocks), dim3(BLOCK), 0, g->stream,
+                           (const double *)g->d_stage, (const int *)g->d_cell_perm, g->ni,
+                           g->cfg.gamma, g->d_U0);
+        g->u0_valid = 1;
+        g->dt_valid = 0;
+    }
     HIPCALL(hipGetLastError());
     return 0;
 }
@@ -888,7 +921,6 @@ extern "C" int teal_gpu_upload(TealGpu *g, const double *primitive)
                                          : upload_primitive(g, g->d32, primitive);
     if (rc) return rc;
     HIPCALL(hipStreamSynchronize(g->stream));
-    g->u0_valid = 0;
     return 0;
 }
 
@@ -899,6 +931,21 @@ extern "C" int teal_gpu_download(TealGpu *g, double *primitive)
                  ? download_rows(g, g->d64.d_P, g->nc, 5, 5, 0, 5, primitive)
                  : download_rows(g, g->d32.d_P, g->nc, 5, 5, 0, 5, primitive);
     if (rc) return rc;
+    if (g->precision == TEAL_MIXED && g->u0_valid) {
+        // inner states from the FP64 solution U instead of its FP32 copy
+        hipLaunchKernelGGL(k_prim_from_u, dim3(g->nblocks), dim3(BLOCK), 0, g->stream,
+                           (const double *)g->d_U0, (const int *)g->d_cell_perm, g->ni,
+                           g->cfg.gamma, g->d_stage);
+        HIPCALL(hipGetLastError());
+        std::vector<double> tmp((size_t)5 * g->ni);
+        HIPCALL(hipMemcpyAsync(tmp.data(), g->d_stage, sizeof(double) * 5 * (size_t)g->ni,
+                               hipMemcpyDeviceToHost, g->stream));
+        HIPCALL(hipStreamSynchronize(g->stream));
+        for (int d = 0; d < g->ni; d++) {
+            int c = g->cell_perm[d];
+            memcpy(&primitive[(size_t)c * 5], &tmp[(size_t)c * 5], sizeof(double) * 5);
+        }
+    }
     fill_periodic_ghosts(g, primitive, 5);
     return 0;
 }
@@ -936,9 +983,10 @@ template <typename S>
 static int explicit_step(TealGpu *g, DevData<S> &d, double max_step, double courant,
                          const double *alpha, int num_stages)
 {
-    if (!g->u0_valid) {
-        if (teal_timestep_pass(g, d, 1, nullptr)) return -1;
+    if (!g->u0_valid || !g->dt_valid) {  // U0 = prim2cons(P) unless the upload provided it
+        if (teal_timestep_pass(g, d, !g->u0_valid, nullptr)) return -1;
         g->u0_valid = 1;
+        g->dt_valid = 1;
     }
     hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, g->stream, g->d_scalars, courant,
                        max_step);
@@ -1010,7 +1058,7 @@ extern "C" int teal_gpu_run_lserk(TealGpu *g, int steps, double *time, double co
     }
     const double *alpha = lserk_alpha[time_order - 1][num_stages - 1];
     hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(1), 0, g->stream, g->d_scalars, *time);
-    g->u0_valid = 0;
+    g->dt_valid = 0;
     for (int i = 0; i < steps; i++) {
         int rc = (g->precision == TEAL_FP64)
                      ? explicit_step(g, g->d64, INFINITY, courant, alpha, num_stages)
