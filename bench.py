@@ -67,6 +67,25 @@ def byte_model(ni, nc, nf, mixed, viscous=True, vk=False):
             "per_cell_update": (b_step + 3 * (b_grad + b_lim + b_flux)) / ni}
 
 
+def measured_traffic(precision, path=os.path.join(ROOT, "profiles", "hbm_traffic_tgv128.json")):
+    """HBM bytes per launch of the fused stage kernel (RK stages and last stages weighted by
+    their dispatch counts) from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this
+    workload (calibrated against a copy of known size), or None."""
+    try:
+        with open(path) as f:
+            data = json.load(f)["kernels"]
+    except (OSError, ValueError, KeyError):
+        return None
+    typ = "float" if precision == "mixed" else "double"
+    rows = [v for k, v in data.items() if k.startswith(f"k_flux_block<{typ},")]
+    n = sum(r["dispatches"] for r in rows)
+    if not n:
+        return None
+    return {"bytes": sum(r["traffic_bytes"] * r["dispatches"] for r in rows) / n,
+            "average_ns": sum(r["average_ns"] * r["dispatches"] for r in rows) / n,
+            "source": os.path.relpath(path, ROOT)}
+
+
 def make_gpu(mesh, precision):
     g = teal_amd.Gpu(mesh, precision)
     g.configure(space_order=2, flux="hllc", viscous=1, limiter="minmod", navierstokes=1,
@@ -256,6 +275,7 @@ def main():
             "fused", "grad_ms", "grad_gbs", "stage_ms", "stage_gbs", "face_ms", "face_gbs"]
     timed = [k for k in ("grad", "face", "stage") if k + "_gbs" in mx]
     dom = max(timed, key=lambda k: mx[k + "_ms"])  # dominant kernel
+    traffic = measured_traffic("mixed") if (dom == "stage" and mx["fused"]) else None
     line = {
         "metric": "RK cell-updates/s, TGV 128^3, mixed vs FP64 GPU at stated L2 error; %HBM",
         "value": mx["cell_updates_per_s"],
@@ -286,7 +306,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": mx[dom + "_gbs"] / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic["bytes"] if traffic else None,
+            "traffic_source": traffic["source"] if traffic else None,
+            "traffic_rate_gbs": (traffic["bytes"] / (traffic["average_ns"] * 1e-9) / 1e9
+                                 if traffic else None),
             "algorithmic_bytes_per_launch": mx[dom + "_bytes"],
             "mean_launch_ms": mx[dom + "_ms"],
         },

@@ -110,9 +110,19 @@ int teal_gpu_run_lserk(TealGpu *gpu, int steps, double *time, double courant, in
 int teal_gpu_timing(TealGpu *gpu, int capacity);
 double teal_gpu_timing_ms(TealGpu *gpu, int kernel, int *launches);
 
-// Device-layout facts for measurement: counts and the algorithmic bytes moved per launch
-// of the gradient/limiter kernel and of the flux kernel (see the byte model in the docs).
+// Device-layout counts for measurement: inner cells, cells, faces, inner faces, offset of
+// the periodic faces, maximum faces per cell.
 int teal_gpu_info(TealGpu *gpu, long *counts /* [6]: ni, nc, nf, nfi, nfb, W */);
+
+// Precision of the device mirror behind an Equations (no counterpart in the reference,
+// whose path is FP64 only): TEAL_FP64 (the default) or TEAL_MIXED.  Takes effect at the
+// next device call; the mirror is rebuilt when the precision changes.
+struct Equations;
+void teal_equations_set_precision(struct Equations *eqns, int precision);
+
+// The configured device mirror of an Equations (created on first use; teal_error when one of
+// its plugins has no device counterpart).  For callers that drive the C-ABI above directly.
+TealGpu *teal_equations_device(const struct Equations *eqns);
 
 #ifdef __cplusplus
 }

@@ -11,16 +11,30 @@
  * -ffp-contract=off), so results are bit-identical to the reference wherever
  * the reference itself is deterministic.
  *
- * Pinning (what checks each part):
- *   - mesh create, equations path, integrators, GMRES: bit-for-bit against the
- *     reference's own sources compiled into oracle/_ref (see oracle/Makefile),
- *     plus closed forms (free-stream preservation, linear-field gradients,
- *     flux consistency, Toro star states, dense GMRES vs numpy).
- *   - dual graph (METIS_MeshToDual, third-party, absent): restated from METIS
- *     5.1.0 CreateGraphDual/FindCommonElements; checked by brute-force face
- *     adjacency and the reference's own mesh_validate (built in oracle/_ref).
- *   - mesh geometry (src/mesh/generate.c, not buildable without METIS):
- *     reference mesh_validate invariants, closed forms on uniform hexes.
+ * Pinning (what checks each part; tests/ holds the checks):
+ *   - mesh create, equations path (boundary, gradient, limiter, fluxes, viscous
+ *     flux, derivative, time step, residual), integrators (euler, lserk, every
+ *     order/stage pair, implicit_euler) and GMRES: bit-for-bit against the
+ *     reference's own sources compiled into oracle/_ref (see oracle/Makefile)
+ *     and against the golden fixtures written from that build
+ *     (tests/golden/*.npz); plus closed forms (flux consistency, supersonic
+ *     upwinding, rotation and mirror invariance, Toro's exact-Riemann star
+ *     states, BC invariants, viscous stress and Fourier heat flux, LSQ
+ *     exactness on linear fields and the reference's convergence-order test,
+ *     limiter bounds, free-stream preservation, discrete conservation, the
+ *     uniform-hex time step).
+ *   - dual graph (METIS_MeshToDual; METIS is third-party and absent here, its
+ *     version unpinned by the reference Makefile): restated from the published
+ *     METIS 5.1.0 CreateGraphDual / FindCommonElements rule; checked by brute
+ *     force on small meshes and by the reference's own mesh_validate.
+ *   - mesh generation (src/mesh/generate.c needs metis.h, so the reference
+ *     build cannot compile it): the reference's mesh_validate, closed-form hex
+ *     geometry, closed cells and volume sums on jittered tets, the BFS property
+ *     of the cell numbering, the face order.
+ *   - parity unpinned: the neighbour order real METIS would return (hence the
+ *     BFS tie-breaking and the exact cell/face numbering of mesh_generate)
+ *     cannot be compared with METIS itself; it follows the published
+ *     algorithm and satisfies every invariant above.
  */
 #pragma once
 

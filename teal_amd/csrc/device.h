@@ -58,7 +58,7 @@ struct Kern {
                           fv::Topo t, fv::BlockTopo bt, fv::Geom<S> g, fv::Phys ph, const S *P,
                           const S *GP, S *Pout, fv::StageArgs a);
     static void timestep_u0(hipStream_t s, fv::Topo t, fv::Geom<S> g, fv::Phys ph,
-                            const S *P, double *U0, fv::Scalars *sc, int write_u0);
+                            const S *P, double *U0, double *dtpart, int write_u0);
     // host layout (reference numbering) <-> device layout (device numbering);
     // perm[d] = reference index of device row d (null: identity)
     static void to_device(hipStream_t s, const double *in, S *out, int num, int in_stride,
@@ -100,6 +100,7 @@ struct TealGpu {
     double *d_U0 = nullptr;
     double *d_R = nullptr;
     double *d_partials = nullptr;
+    double *d_dtpart = nullptr;  // [nblocks] per-block time-step minima
     fv::Scalars *d_scalars = nullptr;
     fv::Scalars *h_scalars = nullptr;
     int timing = 0;

@@ -53,37 +53,61 @@ __global__ void k_init_scalars(Scalars *sc, double time)
     sc->error = 0;
 }
 
-__global__ void k_reset_dt(Scalars *sc) { sc->dtmin_bits = 0x7ff0000000000000ULL; }
+// dt minimum over the per-block minima (src/equations/timestep.c:22-29: a min, so the order
+// does not matter); begin != 0 also sets step0 = courant * dt and step = min(step0, max_step)
+// for the step about to run (src/simulation/advance.c:92-93)
+constexpr int DT_THREADS = 1024;
 
-// step0 = courant * min dt (advance.c:92-93); the accumulator is reset for the next step
-__global__ void k_step_begin(Scalars *sc, double courant, double max_step)
+__global__ __launch_bounds__(DT_THREADS) void k_dt_min(Scalars *sc, const double *dtpart, int n,
+                                                      int begin, double courant, double max_step)
 {
-    double dtmin = __longlong_as_double((long long)sc->dtmin_bits);
-    sc->step0 = courant * dtmin;
-    sc->step = fmin(sc->step0, max_step);
-    sc->dtmin_bits = 0x7ff0000000000000ULL;
+    __shared__ double red[DT_THREADS / 64];
+    double dt = __longlong_as_double(0x7ff0000000000000LL);
+    for (int b = threadIdx.x; b < n; b += DT_THREADS) {
+        dt = fmin(dt, dtpart[b]);
+    }
+    for (int off = 32; off > 0; off >>= 1) dt = fmin(dt, __shfl_xor(dt, off));
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6] = dt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < DT_THREADS / 64; w++) dt = fmin(dt, red[w]);
+        sc->dtmin_bits = (unsigned long long)__double_as_longlong(dt);
+        if (begin) {
+            sc->step0 = courant * dt;
+            sc->step = fmin(sc->step0, max_step);
+        }
+    }
 }
 
-// residual = sqrt(sum V R^2 / sum V) from per-block partials in a fixed order
-// (src/equations/residual.c:8-33), then time += step (advance.c:111)
-__global__ void k_step_end(Scalars *sc, const double *partials, int nblocks, double sum_volume,
-                           int advance_time)
+// residual = sqrt(sum V R^2 / sum V) from per-block partials [nblocks][5] in a fixed order
+// (src/equations/residual.c:8-33), then time += step (advance.c:111).  Thread t of the
+// STEP_END_THREADS = 5 x 128 sums variable t % 5 of blocks t / 5, t / 5 + 128, ... (coalesced
+// reads), then a fixed tree over the 128 partial sums.
+constexpr int STEP_END_THREADS = 640;
+
+__global__ __launch_bounds__(STEP_END_THREADS) void k_step_end(Scalars *sc, const double *partials,
+                                                               int nblocks, double sum_volume,
+                                                               int advance_time)
 {
-    __shared__ double red[BLOCK][5];
-    double s[5] = {0, 0, 0, 0, 0};
-    for (int b = threadIdx.x; b < nblocks; b += BLOCK) {
-        for (int v = 0; v < 5; v++) s[v] += partials[(b * 5) + v];
+    __shared__ double red[STEP_END_THREADS];
+    const int t = threadIdx.x;
+    const int lanes = STEP_END_THREADS / 5;
+    double s = 0;
+    for (long e = t; e < (long)nblocks * 5; e += STEP_END_THREADS) {
+        s += partials[e];
     }
-    for (int v = 0; v < 5; v++) red[threadIdx.x][v] = s[v];
+    red[t] = s;
     __syncthreads();
-    for (int h = BLOCK / 2; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h) {
-            for (int v = 0; v < 5; v++) red[threadIdx.x][v] += red[threadIdx.x + h][v];
+    for (int h = lanes / 2; h > 0; h >>= 1) {
+        if (t < h * 5) {
+            red[t] += red[t + (h * 5)];
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        for (int v = 0; v < 5; v++) sc->residual[v] = sqrt(red[0][v] / sum_volume);
+    if (t == 0) {
+        for (int v = 0; v < 5; v++) sc->residual[v] = sqrt(red[v] / sum_volume);
         if (advance_time) {
             sc->time += sc->step;
         }
@@ -502,6 +526,7 @@ extern "C" TealGpu *teal_gpu_create(const Mesh *mesh, int precision)
          hipMalloc((void **)&g->d_U0, sizeof(double) * 5 * (size_t)g->ni) != hipSuccess ||
          hipMalloc((void **)&g->d_R, sizeof(double) * 5 * (size_t)g->ni) != hipSuccess ||
          hipMalloc((void **)&g->d_partials, sizeof(double) * 5 * (size_t)g->nblocks) != hipSuccess ||
+         hipMalloc((void **)&g->d_dtpart, sizeof(double) * (size_t)g->nblocks) != hipSuccess ||
          hipMalloc((void **)&g->d_scalars, sizeof(Scalars)) != hipSuccess ||
          hipHostMalloc((void **)&g->h_scalars, sizeof(Scalars), hipHostMallocDefault) != hipSuccess)) {
         teal_gpu_set_error("teal_gpu_create: device allocation failed");
@@ -531,7 +556,8 @@ extern "C" void teal_gpu_destroy(TealGpu *g)
     }
     g->d64.release();
     g->d32.release();
-    void *ptrs[] = {g->d_stage, g->d_U0, g->d_R, g->d_partials, g->d_scalars, g->d_cell_perm};
+    void *ptrs[] = {g->d_stage, g->d_U0, g->d_R, g->d_partials, g->d_dtpart, g->d_scalars,
+                    g->d_cell_perm};
     for (void *p : ptrs) {
         if (p) {
             (void)hipFree(p);
@@ -659,6 +685,7 @@ int teal_residual_pass(TealGpu *g, DevData<S> &d, int mode, double alpha, int la
     a.alpha = alpha;
     a.scalars = g->d_scalars;
     a.partials = g->d_partials;
+    a.dtpart = g->d_dtpart;
     a.keep_u = (g->precision == TEAL_MIXED);
     int kmode = (mode == 0) ? 0 : (last ? 2 : 1);
     if (d.fused) {  // one kernel: face fluxes in LDS, accumulation, stage update into d_P2
@@ -866,11 +893,12 @@ extern "C" int teal_gpu_limiter(TealGpu *g, const double *primitive, double *gra
 template <typename S>
 int teal_timestep_pass(TealGpu *g, DevData<S> &d, int write_u0, double *out)
 {
-    hipLaunchKernelGGL(k_reset_dt, dim3(1), dim3(1), 0, g->stream, g->d_scalars);
-    Kern<S>::timestep_u0(g->stream, g->topo(), d.geom(), g->phys(), d.d_P, g->d_U0, g->d_scalars,
+    Kern<S>::timestep_u0(g->stream, g->topo(), d.geom(), g->phys(), d.d_P, g->d_U0, g->d_dtpart,
                          write_u0);
     HIPCALL(hipGetLastError());
     if (out) {
+        hipLaunchKernelGGL(k_dt_min, dim3(1), dim3(DT_THREADS), 0, g->stream, g->d_scalars,
+                           (const double *)g->d_dtpart, g->nblocks, 0, 0.0, 0.0);
         HIPCALL(hipMemcpyAsync(g->h_scalars, g->d_scalars, sizeof(Scalars), hipMemcpyDeviceToHost,
                                g->stream));
         HIPCALL(hipStreamSynchronize(g->stream));
@@ -903,7 +931,7 @@ extern "C" int teal_gpu_residual(TealGpu *g, const double *derivative, double *r
     const double *vol64 = (g->precision == TEAL_FP64) ? g->d64.d_vol64 : g->d32.d_vol64;
     hipLaunchKernelGGL(k_residual_partials, dim3(g->nblocks), dim3(BLOCK), 0, g->stream, g->d_R,
                        vol64, g->ni, g->d_partials);
-    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(BLOCK), 0, g->stream, g->d_scalars,
+    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(STEP_END_THREADS), 0, g->stream, g->d_scalars,
                        (const double *)g->d_partials, g->nblocks, g->sum_volume, 0);
     HIPCALL(hipGetLastError());
     HIPCALL(hipMemcpyAsync(g->h_scalars, g->d_scalars, sizeof(Scalars), hipMemcpyDeviceToHost,
@@ -988,12 +1016,12 @@ static int explicit_step(TealGpu *g, DevData<S> &d, double max_step, double cour
         g->u0_valid = 1;
         g->dt_valid = 1;
     }
-    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, g->stream, g->d_scalars, courant,
-                       max_step);
+    hipLaunchKernelGGL(k_dt_min, dim3(1), dim3(DT_THREADS), 0, g->stream, g->d_scalars,
+                       (const double *)g->d_dtpart, g->nblocks, 1, courant, max_step);
     for (int s = 0; s < num_stages; s++) {
         if (teal_residual_pass(g, d, 1, alpha[s + 1], s == num_stages - 1)) return -1;
     }
-    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(BLOCK), 0, g->stream, g->d_scalars,
+    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(STEP_END_THREADS), 0, g->stream, g->d_scalars,
                        (const double *)g->d_partials, g->nblocks, g->sum_volume, 1);
     HIPCALL(hipGetLastError());
     return 0;

@@ -41,6 +41,13 @@ namespace fv {
 
 constexpr int W = 6;        // MAX_CELL_FACES (src/mesh.h:8-12)
 constexpr int BLOCK = 256;  // 4 wavefronts of 64
+// threads of the fused stage kernel per block of BLOCK cells: the face phase spreads a block's
+// ~900 faces over them, the cell phase uses the first BLOCK.  Measured on TGV 128^3: 256 beats
+// 512 (0.62 vs 0.75 ms FP64): more resident waves only enlarge the gathered working set.
+#ifndef TEAL_FLUX_THREADS
+#define TEAL_FLUX_THREADS 256
+#endif
+constexpr int FLUX_THREADS = TEAL_FLUX_THREADS;
 constexpr int NUM_XCD = 8;
 
 struct Topo {
@@ -368,6 +375,7 @@ struct StageArgs {
     double alpha;         // stage coefficient alpha[i+1]
     Scalars *scalars;     // step, dt reduction
     double *partials;     // MODE 2: [num_blocks][5] residual partial sums
+    double *dtpart;       // MODE 2: [num_blocks] minima of the local time step
     int keep_u;           // MODE 2: next U0 = U (mixed) instead of prim2cons(P_new) (FP64)
 };
 
@@ -375,7 +383,7 @@ struct StageArgs {
 // R = -R/V (derivative.c:253-258); MODE 0 writes R; MODE 1 writes P_new = cons2prim(U0 +
 // (R alpha) step) (advance.c:103-108); MODE 2 also produces the next step's U0, the dt minimum
 // and the residual partial sums of this workgroup (block-wide: all threads must call it).
-template <typename S, int MODE>
+template <typename S, int MODE, int NT = BLOCK>
 __device__ inline void stage_epilogue(const Topo &t, const Geom<S> &g, const Phys &ph, const Acc<S> &acc,
                                       int c, bool active, int blk, S *__restrict__ Pout,
                                       const StageArgs &a)
@@ -398,7 +406,7 @@ __device__ inline void stage_epilogue(const Topo &t, const Geom<S> &g, const Phy
         }
     }
     if (MODE == 2) {
-        __shared__ double red[BLOCK / 64][6];
+        __shared__ double red[NT / 64][6];
         double dt = __longlong_as_double(0x7ff0000000000000LL);  // +inf
         double sums[5] = {0, 0, 0, 0, 0};
         if (active) {
@@ -436,11 +444,11 @@ __device__ inline void stage_epilogue(const Topo &t, const Geom<S> &g, const Phy
             double bdt = red[0][0];
             double bs[5];
             for (int v = 0; v < 5; v++) bs[v] = red[0][v + 1];
-            for (int w = 1; w < BLOCK / 64; w++) {
+            for (int w = 1; w < NT / 64; w++) {
                 bdt = fmin(bdt, red[w][0]);
                 for (int v = 0; v < 5; v++) bs[v] += red[w][v + 1];
             }
-            atomicMin(&a.scalars->dtmin_bits, (unsigned long long)__double_as_longlong(bdt));
+            a.dtpart[blk] = bdt;
             for (int v = 0; v < 5; v++) a.partials[(blk * 5) + v] = bs[v];
         }
     }
@@ -527,7 +535,7 @@ struct BlockTopo {
 };
 
 template <typename S, int FLUX, int VISC, int ORDER, int MODE>
-__global__ __launch_bounds__(BLOCK) void k_flux_block(Topo t, BlockTopo bt, Geom<S> g, Phys ph,
+__global__ __launch_bounds__(FLUX_THREADS) void k_flux_block(Topo t, BlockTopo bt, Geom<S> g, Phys ph,
                                                       const S *__restrict__ P,
                                                       const S *__restrict__ GP,
                                                       S *__restrict__ Pout, StageArgs a)
@@ -540,7 +548,7 @@ __global__ __launch_bounds__(BLOCK) void k_flux_block(Topo t, BlockTopo bt, Geom
     const int beg = bt.off[blk];
     const int num = bt.off[blk + 1] - beg;
     int err = 0;
-    for (int j = threadIdx.x; j < num; j += BLOCK) {
+    for (int j = threadIdx.x; j < num; j += FLUX_THREADS) {
         S fv[5], fc[5];
         face_flux<S, FLUX, VISC, ORDER>(t, g, ph, P, GP, bt.face[beg + j], fv, fc, &err);
         if (VISC) {
@@ -555,7 +563,7 @@ __global__ __launch_bounds__(BLOCK) void k_flux_block(Topo t, BlockTopo bt, Geom
     }
     __syncthreads();
     const int c = (blk * BLOCK) + threadIdx.x;
-    const bool active = c < t.ni;
+    const bool active = (threadIdx.x < BLOCK) && (c < t.ni);
     Acc<S> acc;
     acc.zero();
     if (active) {
@@ -564,14 +572,16 @@ __global__ __launch_bounds__(BLOCK) void k_flux_block(Topo t, BlockTopo bt, Geom
             for (int v = 0; v < 5; v++) x[v] = lds[(((part ? 5 : 0) + v) * mf) + j];
         });
     }
-    stage_epilogue<S, MODE>(t, g, ph, acc, c, active, blk, Pout, a);
+    stage_epilogue<S, MODE, FLUX_THREADS>(t, g, ph, acc, c, active, blk, Pout, a);
 }
 
 // ---------------------------------------------------------------- time step + U0
+// local time step (src/equations/timestep.c:7-30) and, if asked, U0 = prim2cons(P)
+// (advance.c:95-98); dtpart[block] = the block's minimum
 template <typename S>
 __global__ __launch_bounds__(BLOCK) void k_timestep_u0(Topo t, Geom<S> g, Phys ph,
                                                        const S *__restrict__ P, double *U0,
-                                                       Scalars *sc, int write_u0)
+                                                       double *dtpart, int write_u0)
 {
     int c = (blockIdx.x * BLOCK) + threadIdx.x;
     double dt = __longlong_as_double(0x7ff0000000000000LL);
@@ -588,9 +598,15 @@ __global__ __launch_bounds__(BLOCK) void k_timestep_u0(Topo t, Geom<S> g, Phys p
             for (int v = 0; v < 5; v++) U0[(v * t.ni) + c] = u[v];
         }
     }
+    __shared__ double red[BLOCK / 64];
     for (int off = 32; off > 0; off >>= 1) dt = fmin(dt, __shfl_xor(dt, off));
     if ((threadIdx.x & 63) == 0) {
-        atomicMin(&sc->dtmin_bits, (unsigned long long)__double_as_longlong(dt));
+        red[threadIdx.x >> 6] = dt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < BLOCK / 64; w++) dt = fmin(dt, red[w]);
+        dtpart[blockIdx.x] = dt;
     }
 }
 

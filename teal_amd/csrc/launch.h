@@ -80,8 +80,8 @@ static void launch_block(hipStream_t s, size_t lds, Topo t, BlockTopo bt, Geom<S
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess) {
         granted = lds;
     }
-    hipLaunchKernelGGL((k_flux_block<S, FLUX, VISC, ORDER, MODE>), dim3(grid_of(t.ni)), dim3(BLOCK),
-                       lds, s, t, bt, g, ph, P, GP, Pout, a);
+    hipLaunchKernelGGL((k_flux_block<S, FLUX, VISC, ORDER, MODE>), dim3(grid_of(t.ni)),
+                       dim3(FLUX_THREADS), lds, s, t, bt, g, ph, P, GP, Pout, a);
 }
 
 template <typename S, int FLUX, int VISC, int ORDER>
@@ -197,10 +197,10 @@ void Kern<S>::accum(hipStream_t s, int visc, int conv, int mode, fv::Topo t, fv:
 
 template <typename S>
 void Kern<S>::timestep_u0(hipStream_t s, fv::Topo t, fv::Geom<S> g, fv::Phys ph, const S *P,
-                          double *U0, fv::Scalars *sc, int write_u0)
+                          double *U0, double *dtpart, int write_u0)
 {
     hipLaunchKernelGGL(fv::k_timestep_u0<S>, dim3(fv::grid_of(t.ni)), dim3(fv::BLOCK), 0, s, t, g,
-                       ph, P, U0, sc, write_u0);
+                       ph, P, U0, dtpart, write_u0);
 }
 
 template <typename S>
