@@ -97,6 +97,14 @@ double teal_gpu_implicit_euler(TealGpu *gpu, double *time, double *residual, dou
                                double courant, double newton_tolerance,
                                double krylov_tolerance, int krylov_dimension);
 
+// Dot products of implicit_euler / GMRES (src/core/sync.c:66-79 sums serially in cell
+// order): TEAL_REDUCE_TREE (default) is a deterministic compensated tree reduction across the
+// GPU; TEAL_REDUCE_SERIAL adds in the reference's order on one workgroup, reproducing the
+// reference bit for bit (slow: for verification on small meshes).  No counterpart in the
+// reference.
+enum TealReduction { TEAL_REDUCE_TREE = 0, TEAL_REDUCE_SERIAL = 1 };
+int teal_gpu_set_reductions(TealGpu *gpu, int mode);
+
 // `steps` lserk steps with no host synchronization between them (the inner loop of
 // src/simulation/run.c:68-102 without output); the final time is returned in *time.
 int teal_gpu_run_lserk(TealGpu *gpu, int steps, double *time, double courant, int time_order,

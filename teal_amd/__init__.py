@@ -21,6 +21,7 @@ LP = C.POINTER(C.c_long)
 VP = C.c_void_p
 
 FP64, MIXED = 0, 1
+REDUCE_TREE, REDUCE_SERIAL = 0, 1
 FLUX = {"none": 0, "godunov": 1, "roe": 2, "hll": 3, "hllc": 4, "hlle": 5, "ausmd": 6,
         "ausmdv": 7, "lxf": 8, "central": 9}
 BC = {"symmetry": 1, "slipwall": 1, "supersonic inflow": 2, "supersonic outflow": 3,
@@ -81,6 +82,7 @@ def lib():
         _sig(l, "teal_gpu_lserk", D, VP, DP, DP, D, D, C.c_int, C.c_int)
         _sig(l, "teal_gpu_implicit_euler", D, VP, DP, DP, D, D, D, D, C.c_int)
         _sig(l, "teal_gpu_run_lserk", C.c_int, VP, C.c_int, DP, D, C.c_int, C.c_int)
+        _sig(l, "teal_gpu_set_reductions", C.c_int, VP, C.c_int)
         _sig(l, "teal_gpu_timing", C.c_int, VP, C.c_int)
         _sig(l, "teal_gpu_timing_ms", D, VP, C.c_int, IP)
         _sig(l, "teal_gpu_info", C.c_int, VP, LP)
@@ -167,6 +169,20 @@ class Gpu:
         if np.isnan(step0):
             raise RuntimeError(f"lserk failed: {last_error()}")
         return step0, t.value, np.array(res[:])
+
+    def implicit_euler(self, time, courant, newton_tol=0.1, krylov_tol=0.3, krylov_dim=32,
+                       max_step=np.inf):
+        t = C.c_double(time)
+        res = (C.c_double * 5)()
+        step0 = self.lib.teal_gpu_implicit_euler(self.h, C.byref(t), res, max_step, courant,
+                                                 newton_tol, krylov_tol, krylov_dim)
+        if np.isnan(step0):
+            raise RuntimeError(f"implicit_euler failed: {last_error()}")
+        return step0, t.value, np.array(res[:])
+
+    def set_reductions(self, mode):
+        check(self.lib.teal_gpu_set_reductions(self.h, mode), "set_reductions")
+        return self
 
     def close(self):
         if self.h:

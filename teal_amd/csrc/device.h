@@ -106,6 +106,7 @@ struct TealGpu {
     int timing = 0;
     EventPool ev_flux, ev_grad, ev_face;
     Krylov *krylov = nullptr;
+    int serial_reductions = 0;  // TEAL_REDUCE_SERIAL: GMRES/Newton dot products in reference order
 
     fv::Topo topo() const;
     fv::Phys phys() const;
@@ -117,4 +118,6 @@ int teal_residual_pass(TealGpu *g, DevData<S> &d, int mode, double alpha, int la
 template <typename S>
 int teal_timestep_pass(TealGpu *g, DevData<S> &d, int write_u0, double *out);
 void teal_krylov_release(TealGpu *g);
+// residual = sqrt(sum V R^2 / sum V) of a device derivative [5][ni] (src/equations/residual.c)
+int teal_residual_norm(TealGpu *g, const double *d_derivative, double *residual);
 void teal_device_numbering(TealGpu *g, const Mesh *m);

@@ -919,6 +919,21 @@ extern "C" double teal_gpu_timestep(TealGpu *g, const double *primitive)
     return rc ? NAN : dt;
 }
 
+int teal_residual_norm(TealGpu *g, const double *d_derivative, double *residual)
+{
+    const double *vol64 = (g->precision == TEAL_FP64) ? g->d64.d_vol64 : g->d32.d_vol64;
+    hipLaunchKernelGGL(k_residual_partials, dim3(g->nblocks), dim3(BLOCK), 0, g->stream, d_derivative,
+                       vol64, g->ni, g->d_partials);
+    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(STEP_END_THREADS), 0, g->stream, g->d_scalars,
+                       (const double *)g->d_partials, g->nblocks, g->sum_volume, 0);
+    HIPCALL(hipGetLastError());
+    HIPCALL(hipMemcpyAsync(g->h_scalars, g->d_scalars, sizeof(Scalars), hipMemcpyDeviceToHost,
+                           g->stream));
+    HIPCALL(hipStreamSynchronize(g->stream));
+    memcpy(residual, g->h_scalars->residual, sizeof(double) * 5);
+    return 0;
+}
+
 extern "C" int teal_gpu_residual(TealGpu *g, const double *derivative, double *residual)
 {
     if (!g) {

@@ -207,3 +207,25 @@ def test_device_explicit_stepping_golden(key):
     nb = _abi.mesh_arrays(gpu.mesh)["off_boundary"]
     np.testing.assert_array_equal(gpu.download()[:nb], STEPPING[key + "/state_out"][:nb])
     gpu.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("key", [k for k in STEPPING_KEYS if "implicit" in k])
+def test_device_implicit_golden(key):
+    """Serial (reference-order) reductions: bit-for-bit."""
+    print(f"golden implicit {key}", flush=True)
+    name, scheme, args = parse_key(key)
+    case, _ = CASES[name]
+    gpu = device(case).set_reductions(teal_amd.REDUCE_SERIAL)
+    gpu.upload(STEPPING[key + "/state"])
+    t = C.c_double(0.0)
+    res = (C.c_double * 5)()
+    for i in range(len(STEPPING[key + "/steps"])):
+        s = gpu.lib.teal_gpu_implicit_euler(gpu.h, C.byref(t), res, 1e300, 5.0, args[0], args[1],
+                                            int(args[2]))
+        assert s == STEPPING[key + "/steps"][i], teal_amd.last_error()
+        assert t.value == STEPPING[key + "/times"][i]
+        np.testing.assert_array_equal(np.array(res[:]), STEPPING[key + "/residuals"][i])
+    nb = _abi.mesh_arrays(gpu.mesh)["off_boundary"]
+    np.testing.assert_array_equal(gpu.download()[:nb], STEPPING[key + "/state_out"][:nb])
+    gpu.close()

@@ -1,0 +1,111 @@
+"""Implicit Euler with Jacobian-free Newton-GMRES on the device (src/simulation/advance.c:
+119-195, src/simulation/gmres.c) against the oracle on the same mesh and state.
+
+Every vector operation matches the reference element by element.  With serial reductions
+(the reference's summation order) the FP64 path is bit-for-bit.  With the default tree
+reductions the dot products round differently in the last bit, which the finite-difference
+matvec amplifies (an ulp in R(P1) - R divided by eps ~ 1e-6 |U|): the bound below is the
+measured spread with a margin.  The mixed path evaluates the matvec's residuals in FP32 with
+a finite-difference step of sqrt(FP32 epsilon); both paths stop the Newton iteration at the
+same relative tolerance, so they differ by about that tolerance times the step's change."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import teal_amd
+from cases import RESIDUAL_CASES
+from test_gpu_residual import setup_pair
+
+pytestmark = pytest.mark.gpu
+
+TREE_RTOL = 1e-8   # state (measured: up to 8.5e-10)
+TREE_DT_RTOL = 1e-7  # time step of the next step: a minimum over cells (measured: 1.1e-8)
+MIXED_RTOL = 1e-3
+
+CASES = [RESIDUAL_CASES[i] for i in (1, 3, 12, 13, 14)]
+SOLVERS = [(0.1, 0.3, 32), (1e-3, 1e-2, 10)]
+
+
+def conserved(P, gamma):
+    U = np.empty_like(P)
+    U[:, 0] = P[:, 0]
+    U[:, 1:4] = P[:, :1] * P[:, 1:4]
+    U[:, 4] = P[:, 4] / (gamma - 1) + 0.5 * P[:, 0] * (P[:, 1:4] ** 2).sum(axis=1)
+    return U
+
+
+def relative_error(Ug, Uo):
+    scale = np.abs(Uo).max(axis=0)
+    scale[1:4] = np.abs(Uo[:, 1:4]).max()
+    return np.abs(Ug - Uo).max(axis=0) / scale
+
+
+def run(name, case, kind, solver, precision, serial=False, steps=2, courant=5.0):
+    from oracle_lib import oracle
+
+    om, oe, gpu, arrays, P = setup_pair(case, kind, precision=precision)
+    if serial:
+        gpu.set_reductions(teal_amd.REDUCE_SERIAL)
+    ni, nc, nb = arrays["num_inner"], arrays["num_cells"], arrays["off_boundary"]
+    po = np.ctypeslib.as_array(oracle().oracle_eqns_primitive(oe), shape=(nc * 5,))
+    po[:] = P.ravel()
+    gpu.upload(P)
+    t_o, t_g = C.c_double(0.0), 0.0
+    res_o = (C.c_double * 5)()
+    out = []
+    for _ in range(steps):
+        s_o = oracle().oracle_implicit_euler(oe, C.byref(t_o), res_o, 1e300, courant, *solver)
+        s_g, t_g, r_g = gpu.implicit_euler(t_g, courant, *solver)
+        out.append((s_o, s_g, t_o.value, t_g, np.array(res_o[:]), r_g))
+    Pg = gpu.download()
+    Po = po.reshape(-1, 5)
+    err = relative_error(conserved(Pg[:ni], case.gamma), conserved(Po[:ni], case.gamma))
+    gpu.close()
+    return out, err, Pg[:nb], Po[:nb]
+
+
+@pytest.mark.parametrize("solver", SOLVERS, ids=["default", "tight"])
+@pytest.mark.parametrize("name,case,kind", CASES, ids=[c[0] for c in CASES])
+def test_implicit_euler_fp64_serial_bitwise(name, case, kind, solver):
+    print(f"implicit serial {name} {solver}", flush=True)
+    steps, err, Pg, Po = run(name, case, kind, solver, teal_amd.FP64, serial=True)
+    for s_o, s_g, t_o, t_g, r_o, r_g in steps:
+        assert s_g == s_o and t_g == t_o
+        np.testing.assert_array_equal(r_g, r_o)
+    np.testing.assert_array_equal(Pg, Po)
+
+
+@pytest.mark.parametrize("solver", SOLVERS, ids=["default", "tight"])
+@pytest.mark.parametrize("name,case,kind", CASES, ids=[c[0] for c in CASES])
+def test_implicit_euler_fp64_tree(name, case, kind, solver):
+    print(f"implicit tree {name} {solver}", flush=True)
+    steps, err, _, _ = run(name, case, kind, solver, teal_amd.FP64)
+    assert steps[0][0] == steps[0][1]  # the first time step comes from the identical state
+    for s_o, s_g, t_o, t_g, r_o, r_g in steps:
+        print(f"  dt {s_g} vs {s_o}, residual {r_g} vs {r_o}", flush=True)
+        np.testing.assert_allclose([s_g, t_g], [s_o, t_o], rtol=TREE_DT_RTOL)
+        # the residual norm is |R| of the new state, a small difference quantity after a
+        # large implicit step: state differences of 1e-9 show up as ~1e-4 here
+        np.testing.assert_allclose(r_g, r_o, rtol=1e-3, atol=0)
+    print(f"  state error / max |U| per variable: {err}", flush=True)
+    assert (err <= TREE_RTOL).all(), err
+
+
+@pytest.mark.parametrize("name,case,kind", CASES, ids=[c[0] for c in CASES])
+def test_implicit_euler_mixed(name, case, kind):
+    print(f"implicit mixed {name}", flush=True)
+    steps, err, _, _ = run(name, case, kind, SOLVERS[1], teal_amd.MIXED)
+    print(f"  state error / max |U| per variable: {err}", flush=True)
+    assert (err <= MIXED_RTOL).all(), err
+
+
+def test_implicit_euler_invalid_arguments():
+    name, case, kind = CASES[0]
+    _, _, gpu, _, P = setup_pair(case, kind)
+    gpu.upload(P)
+    t = C.c_double(0.0)
+    res = (C.c_double * 5)()
+    assert np.isnan(gpu.lib.teal_gpu_implicit_euler(gpu.h, C.byref(t), res, 1e300, 5.0, 0.0, 0.3, 32))
+    assert "newton tolerance" in teal_amd.last_error()
+    gpu.close()
