@@ -19,8 +19,11 @@ from test_gpu_residual import setup_pair
 
 pytestmark = pytest.mark.gpu
 
-TREE_RTOL = 1e-8   # state (measured: up to 8.5e-10)
-TREE_DT_RTOL = 1e-7  # time step of the next step: a minimum over cells (measured: 1.1e-8)
+# Tree reductions: the Newton-GMRES iterates follow the reference's only until a last-bit
+# difference, amplified by the finite-difference matvec, changes a Krylov or Newton stopping
+# decision; from there the two solutions agree to the solver tolerances.  Bound: 1e-2 times
+# the Newton tolerance (measured: 5e-11 at tolerance 0.1, 2.2e-6 at 1e-3).
+TREE_FACTOR = 1e-2
 MIXED_RTOL = 1e-3
 
 CASES = [RESIDUAL_CASES[i] for i in (1, 3, 12, 13, 14)]
@@ -81,15 +84,16 @@ def test_implicit_euler_fp64_serial_bitwise(name, case, kind, solver):
 def test_implicit_euler_fp64_tree(name, case, kind, solver):
     print(f"implicit tree {name} {solver}", flush=True)
     steps, err, _, _ = run(name, case, kind, solver, teal_amd.FP64)
+    bound = TREE_FACTOR * solver[0]
     assert steps[0][0] == steps[0][1]  # the first time step comes from the identical state
     for s_o, s_g, t_o, t_g, r_o, r_g in steps:
         print(f"  dt {s_g} vs {s_o}, residual {r_g} vs {r_o}", flush=True)
-        np.testing.assert_allclose([s_g, t_g], [s_o, t_o], rtol=TREE_DT_RTOL)
-        # the residual norm is |R| of the new state, a small difference quantity after a
-        # large implicit step: state differences of 1e-9 show up as ~1e-4 here
-        np.testing.assert_allclose(r_g, r_o, rtol=1e-3, atol=0)
+        np.testing.assert_allclose([s_g, t_g], [s_o, t_o], rtol=bound)
+        # the residual norm is |R| of the new state, a small difference quantity after an
+        # implicit step: it moves by about the Newton tolerance
+        np.testing.assert_allclose(r_g, r_o, rtol=solver[0], atol=0)
     print(f"  state error / max |U| per variable: {err}", flush=True)
-    assert (err <= TREE_RTOL).all(), err
+    assert (err <= bound).all(), err
 
 
 @pytest.mark.parametrize("name,case,kind", CASES, ids=[c[0] for c in CASES])
