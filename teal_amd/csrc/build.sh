@@ -32,6 +32,6 @@ done
 rc=0
 for p in "${pids[@]}"; do wait "$p" || rc=1; done
 [ $rc -eq 0 ] || { echo "build failed" >&2; exit 1; }
-/opt/rocm/bin/hipcc -shared -fPIC --offload-arch="$ARCH" -o "$OUT.tmp" "$OBJ"/*.o -lm
+/opt/rocm/bin/hipcc -shared -fPIC --offload-arch="$ARCH" -o "$OUT.tmp" "$OBJ"/*.o -lm -Wl,--no-undefined
 mv "$OUT.tmp" "$OUT"
 echo "built $OUT"

@@ -31,16 +31,24 @@ struct DevData {
     S *d_F = nullptr;    // [num_faces][10] (split path only)
     S *d_eps2 = nullptr;
     // per-block face lists of the fused kernel (fv::BlockTopo)
-    int *d_boff = nullptr, *d_bflist = nullptr, *d_bslot = nullptr;
+    int *d_boff = nullptr, *d_bslot = nullptr;
+    int2 *d_bflr = nullptr;
+    S *d_barea = nullptr, *d_bbasis = nullptr, *d_boffl = nullptr, *d_boffr = nullptr,
+      *d_bcorr = nullptr;  // face geometry in block-list order (stride num_block_faces)
+    int num_block_faces = 0;
     int max_block_faces = 0;
     int fused = 0;  // 1: residual passes use k_flux_block (its face lists fit in LDS and the
                     // environment does not set TEAL_GPU_SPLIT=1)
 
     int build(const Mesh *mesh, TealGpu *gpu);
-    int build_blocks(const std::vector<int> &inc, int ni);
+    int build_blocks(const std::vector<int> &inc, int ni, const std::vector<int2> &flr,
+                     const std::vector<double> &area, const std::vector<double> &basis,
+                     const std::vector<double> &offl, const std::vector<double> &offr,
+                     const std::vector<double> &corr);
     void release();
     fv::Geom<S> geom() const;
     fv::BlockTopo block_topo() const;
+    fv::Geom<S> block_geom() const;  // cell geometry + face geometry in block-list order
     size_t block_lds() const { return sizeof(S) * 10 * (size_t)max_block_faces; }
 };
 

@@ -394,15 +394,20 @@ int DevData<S>::build(const Mesh *m, TealGpu *gpu)
     HIPCALL(hipMemset(d_F, 0, sizeof(S) * 10 * (size_t)(nf + 1)));
     HIPCALL(hipMalloc((void **)&d_eps2, sizeof(S) * (size_t)ni));
     HIPCALL(hipMemset(d_eps2, 0, sizeof(S) * (size_t)ni));
-    return build_blocks(inc, ni);
+    return build_blocks(inc, ni, flr, area, basis, offl, offr, corr);
 }
 
 // Face lists of the fused stage kernel: block b owns device cells [b*BLOCK, (b+1)*BLOCK); its
 // list holds every face with an incidence in the block, once, in device face order, and
-// slot[k][c] = (list position << 1) | side for incidence k of cell c (-1: none).
+// slot[k][c] = (list position << 1) | side for incidence k of cell c (-1: none).  The face
+// geometry is copied into list order (faces between blocks twice).
 template <typename S>
-int DevData<S>::build_blocks(const std::vector<int> &inc, int ni)
+int DevData<S>::build_blocks(const std::vector<int> &inc, int ni, const std::vector<int2> &flr,
+                             const std::vector<double> &area, const std::vector<double> &basis,
+                             const std::vector<double> &offl, const std::vector<double> &offr,
+                             const std::vector<double> &corr)
 {
+    const int nf = (int)area.size();
     const int nblk = (ni + BLOCK - 1) / BLOCK;
     std::vector<int> off(nblk + 1, 0), list, slot((size_t)W * ni, -1);
     std::vector<int> faces;
@@ -435,10 +440,25 @@ int DevData<S>::build_blocks(const std::vector<int> &inc, int ni)
         off[b + 1] = (int)list.size();
         max_block_faces = std::max(max_block_faces, (int)faces.size());
     }
-    if (list.empty()) {
-        list.push_back(0);
+    const int nj = (int)list.size();
+    const size_t cap = (size_t)std::max(nj, 1);
+    num_block_faces = nj;
+    std::vector<int2> lflr(cap);
+    std::vector<double> larea(cap), lbasis(9 * cap), loffl(3 * cap), loffr(3 * cap), lcorr(4 * cap);
+    for (int j = 0; j < nj; j++) {
+        int f = list[j];
+        lflr[j] = flr[f];
+        larea[j] = area[f];
+        for (int k = 0; k < 9; k++) lbasis[((size_t)k * nj) + j] = basis[((size_t)k * nf) + f];
+        for (int k = 0; k < 3; k++) {
+            loffl[((size_t)k * nj) + j] = offl[((size_t)k * nf) + f];
+            loffr[((size_t)k * nj) + j] = offr[((size_t)k * nf) + f];
+        }
+        for (int k = 0; k < 4; k++) lcorr[((size_t)k * nj) + j] = corr[((size_t)k * nf) + f];
     }
-    if (upload(&d_boff, off) || upload(&d_bflist, list) || upload(&d_bslot, slot)) {
+    if (upload(&d_boff, off) || upload(&d_bslot, slot) || upload(&d_bflr, lflr) ||
+        upload_conv(&d_barea, larea) || upload_conv(&d_bbasis, lbasis) ||
+        upload_conv(&d_boffl, loffl) || upload_conv(&d_boffr, loffr) || upload_conv(&d_bcorr, lcorr)) {
         return -1;
     }
     return 0;
@@ -447,7 +467,13 @@ int DevData<S>::build_blocks(const std::vector<int> &inc, int ni)
 template <typename S>
 BlockTopo DevData<S>::block_topo() const
 {
-    return BlockTopo{d_boff, d_bflist, d_bslot, max_block_faces};
+    return BlockTopo{d_boff, d_bslot, d_bflr, num_block_faces, max_block_faces};
+}
+
+template <typename S>
+Geom<S> DevData<S>::block_geom() const
+{
+    return Geom<S>{d_vol, d_proj, d_eps2, d_w, d_o, d_barea, d_bbasis, d_boffl, d_boffr, d_bcorr};
 }
 
 template <typename S>
@@ -455,7 +481,8 @@ void DevData<S>::release()
 {
     void *ptrs[] = {d_inc, d_nbr, d_flr, d_w, d_o, d_vol, d_proj, d_area, d_basis, d_offl,
                     d_offr, d_corr, d_vol64, d_bleft, d_bright, d_bface, d_bkind, d_bctx,
-                    d_P, d_P2, d_GP, d_F, d_eps2, d_boff, d_bflist, d_bslot};
+                    d_P, d_P2, d_GP, d_F, d_eps2, d_boff, d_bslot, d_bflr, d_barea, d_bbasis,
+                    d_boffl, d_boffr, d_bcorr};
     for (void *p : ptrs) {
         if (p) {
             (void)hipFree(p);
@@ -691,7 +718,8 @@ int teal_residual_pass(TealGpu *g, DevData<S> &d, int mode, double alpha, int la
     if (d.fused) {  // one kernel: face fluxes in LDS, accumulation, stage update into d_P2
         bool timed = g->timing && g->ev_flux.begin(g->stream);
         int rc = Kern<S>::flux_block(g->stream, g->cfg.flux, g->cfg.viscous, g->cfg.space_order, kmode,
-                                     d.block_lds(), g->topo(), d.block_topo(), d.geom(), g->phys(),
+                                     d.block_lds(), g->topo(), d.block_topo(), d.block_geom(),
+                                     g->phys(),
                                      d.d_P, d.d_GP, d.d_P2, a);
         if (timed) {
             g->ev_flux.end(g->stream);

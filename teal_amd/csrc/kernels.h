@@ -40,14 +40,20 @@
 namespace fv {
 
 constexpr int W = 6;        // MAX_CELL_FACES (src/mesh.h:8-12)
-constexpr int BLOCK = 256;  // 4 wavefronts of 64
-// threads of the fused stage kernel per block of BLOCK cells: the face phase spreads a block's
-// ~900 faces over them, the cell phase uses the first BLOCK.  Measured on TGV 128^3: 256 beats
-// 512 (0.62 vs 0.75 ms FP64): more resident waves only enlarge the gathered working set.
+// cells per workgroup (and threads of the per-cell kernels)
+#ifndef TEAL_BLOCK
+#define TEAL_BLOCK 256
+#endif
+constexpr int BLOCK = TEAL_BLOCK;
+// threads of the fused stage kernel per block of BLOCK cells (>= BLOCK): the face phase
+// spreads a block's faces over them, the cell phase uses the first BLOCK.  Measured on TGV
+// 128^3 with 256-cell blocks: 256 threads beat 512 (0.62 vs 0.75 ms FP64), more resident
+// waves only enlarge the gathered working set.
 #ifndef TEAL_FLUX_THREADS
-#define TEAL_FLUX_THREADS 256
+#define TEAL_FLUX_THREADS TEAL_BLOCK
 #endif
 constexpr int FLUX_THREADS = TEAL_FLUX_THREADS;
+static_assert(FLUX_THREADS >= BLOCK && FLUX_THREADS % 64 == 0, "stage kernel threads");
 constexpr int NUM_XCD = 8;
 
 struct Topo {
@@ -526,12 +532,16 @@ __global__ __launch_bounds__(BLOCK) void k_accum(Topo t, Geom<S> g, Phys ph, con
 
 // Per-block face lists of the fused kernel: the workgroup of device cells
 // [b*BLOCK, (b+1)*BLOCK) computes the fluxes of every face touching them into LDS, then each
-// cell sums its faces from there.  slot = position in the block list << 1 | side.
+// cell sums its faces from there.  The faces of block b are list entries [off[b], off[b+1]);
+// a face between two blocks appears in both lists, and the face geometry (Geom face arrays,
+// stride nj) is stored in list order, so each block streams its faces contiguously.
+// slot = position in the block's list << 1 | side.
 struct BlockTopo {
-    const int *__restrict__ off;   // [num_blocks + 1]
-    const int *__restrict__ face;  // device face index per list entry
-    const int *__restrict__ slot;  // [W][ni]
-    int max_faces;                 // longest list (LDS rows)
+    const int *__restrict__ off;    // [num_blocks + 1]
+    const int *__restrict__ slot;   // [W][ni]
+    const int2 *__restrict__ flr;   // [nj] left, right (~ghost row) of each list entry
+    int nj;                         // list entries
+    int max_faces;                  // longest list (LDS rows)
 };
 
 template <typename S, int FLUX, int VISC, int ORDER, int MODE>
@@ -548,9 +558,12 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux_block(Topo t, BlockTopo b
     const int beg = bt.off[blk];
     const int num = bt.off[blk + 1] - beg;
     int err = 0;
+    Topo tf = t;  // face arrays in list order
+    tf.nf = bt.nj;
+    tf.flr = bt.flr;
     for (int j = threadIdx.x; j < num; j += FLUX_THREADS) {
         S fv[5], fc[5];
-        face_flux<S, FLUX, VISC, ORDER>(t, g, ph, P, GP, bt.face[beg + j], fv, fc, &err);
+        face_flux<S, FLUX, VISC, ORDER>(tf, g, ph, P, GP, beg + j, fv, fc, &err);
         if (VISC) {
             for (int v = 0; v < 5; v++) lds[(v * mf) + j] = fv[v];
         }
