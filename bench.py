@@ -197,6 +197,55 @@ def cpu_baseline(n, budget_s=20.0):
                       f"{secs:.1f} s"}
 
 
+def tet_workload(args, n=96, seed=2024):
+    """Euler on n^3 hexes split into 6 tets each, interior nodes jittered by +-0.2 h (seeded),
+    uniform Mach 2 flow (rho = 1, p = 1/gamma, u = 2) plus a Gaussian density bump of
+    amplitude 0.1 (seeded centre, sigma 0.1), supersonic inflow at x = 0 and supersonic
+    outflow elsewhere; LSQ + Barth-Jespersen, LSERK order 2 / 3 stages, CFL 0.5.
+    Prints one JSON line with FP64 and mixed cell-updates/s for Roe and AUSMD."""
+    lib = teal_amd.lib()
+    t0 = time.perf_counter()
+    mesh = lib.mesh_create_tets(n, 0.2, seed)
+    lib.mesh_generate(mesh)
+    a = _abi.mesh_arrays(mesh)
+    nc, ni, nf = a["num_cells"], a["num_inner"], a["num_faces"]
+    print(f"# tet mesh: {ni} cells, {nf} faces, built in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    free = (1.0, 2.0, 0.0, 0.0, 1.0 / GAMMA)
+    x = a["center"][:nc]
+    x0 = np.random.default_rng(seed).uniform(0.3, 0.7, 3)
+    P0 = np.tile(np.array(free), (nc, 1))
+    P0[:, 0] += 0.1 * np.exp(-((x - x0) ** 2).sum(axis=1) / (2 * 0.1 ** 2))
+    P0[ni:] = 0
+    bcs = {"left": ("supersonic inflow", free)}
+    bcs.update({s: ("supersonic outflow", None) for s in ("right", "bottom", "top", "back", "front")})
+    out = {}
+    for flux in ("roe", "ausmd"):
+        for name, prec in (("fp64", teal_amd.FP64), ("mixed", teal_amd.MIXED)):
+            g = teal_amd.Gpu(mesh, prec)
+            g.configure(space_order=2, flux=flux, viscous=0, limiter="minmod", navierstokes=0,
+                        gamma=GAMMA, bcs=bcs)
+            r = timed_run(g, P0, args.steps, args.warmup, time_kernels=True)
+            g.close()
+            ups = ni * args.steps / r["seconds"]
+            out[f"{flux}_{name}"] = {"cell_updates_per_s": ups, "ms_per_step": 1e3 * r["seconds"] / args.steps,
+                                     "grad_ms": r["grad_ms"], "stage_ms": r["stage_ms"]}
+            print(f"# tet {flux} {name}: {1e3 * r['seconds'] / args.steps:.3f} ms/step, {ups:.4g} "
+                  f"cell-updates/s; grad {r['grad_ms']:.3f} ms, stage {r['stage_ms']:.3f} ms",
+                  file=sys.stderr, flush=True)
+    best = out["roe_mixed"]
+    print(json.dumps({
+        "metric": "RK cell-updates/s, Euler on a jittered tet mesh, mixed and FP64 GPU",
+        "value": best["cell_updates_per_s"], "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": best["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 flux/gradient/geometry + f64 update (mixed)", "data": "synthetic",
+        "config": {"workload": f"Euler Mach-2 bump, {n}^3 hexes x 6 jittered tets ({ni} cells), "
+                               "Roe, LSQ + Barth-Jespersen, LSERK 2/3, CFL 0.5",
+                   "cells": ni, "faces": nf},
+        "runs": out}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -208,6 +257,9 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE/"
                          "WRITE_SIZE against a kernel of known traffic)")
+    ap.add_argument("--workload", choices=["tgv", "tet"], default="tgv",
+                    help="tgv: the NS Taylor-Green vortex (the headline metric); tet: Euler "
+                         "Mach-2 bump on the jittered tetrahedral mesh (Roe and AUSMD)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
@@ -225,6 +277,9 @@ def main():
                           "launches": 3}))
         return
 
+    if args.workload == "tet":
+        tet_workload(args)
+        return
     n = args.n
     mesh = teal_amd.mesh_box((n, n, n), (0.0, 0.0, 0.0), (2 * np.pi,) * 3, (1, 1, 1))
     a = _abi.mesh_arrays(mesh)
@@ -239,6 +294,7 @@ def main():
         g = make_gpu(mesh, prec)
         gpus[name] = g
         r = timed_run(g, P0, args.steps, args.warmup, time_kernels=True)
+        r["layout"] = g.info()
         r["ms_per_step"] = 1e3 * r["seconds"] / args.steps
         r["cell_updates_per_s"] = ni * args.steps / r["seconds"]
         model = byte_model(ni, nc, nf, mixed=(prec == teal_amd.MIXED))
@@ -272,7 +328,7 @@ def main():
 
     mx, fp = results["mixed"], results["fp64"]
     KEYS = ["cell_updates_per_s", "ms_per_step", "hbm_fraction_step", "bytes_per_cell_update",
-            "fused", "grad_ms", "grad_gbs", "stage_ms", "stage_gbs", "face_ms", "face_gbs"]
+            "fused", "layout", "grad_ms", "grad_gbs", "stage_ms", "stage_gbs", "face_ms", "face_gbs"]
     timed = [k for k in ("grad", "face", "stage") if k + "_gbs" in mx]
     dom = max(timed, key=lambda k: mx[k + "_ms"])  # dominant kernel
     traffic = measured_traffic("mixed") if (dom == "stage" and mx["fused"]) else None

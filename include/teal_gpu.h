@@ -118,9 +118,11 @@ int teal_gpu_run_lserk(TealGpu *gpu, int steps, double *time, double courant, in
 int teal_gpu_timing(TealGpu *gpu, int capacity);
 double teal_gpu_timing_ms(TealGpu *gpu, int kernel, int *launches);
 
-// Device-layout counts for measurement: inner cells, cells, faces, inner faces, offset of
-// the periodic faces, maximum faces per cell.
-int teal_gpu_info(TealGpu *gpu, long *counts /* [6]: ni, nc, nf, nfi, nfb, W */);
+// Device-layout facts for measurement: inner cells, cells, faces, inner faces, offset of the
+// periodic faces, maximum faces per cell, whether residual passes run the fused stage kernel,
+// and whether it reads the centre form of the face geometry (offsets and corrections
+// recomputed from face and cell centres).
+int teal_gpu_info(TealGpu *gpu, long *counts /* [8]: ni, nc, nf, nfi, nfb, W, fused, centre */);
 
 // Precision of the device mirror behind an Equations (no counterpart in the reference,
 // whose path is FP64 only): TEAL_FP64 (the default) or TEAL_MIXED.  Takes effect at the

@@ -184,6 +184,13 @@ class Gpu:
         check(self.lib.teal_gpu_set_reductions(self.h, mode), "set_reductions")
         return self
 
+    def info(self):
+        c = (C.c_long * 8)()
+        check(self.lib.teal_gpu_info(self.h, c), "info")
+        keys = ("num_inner", "num_cells", "num_faces", "num_inner_faces", "off_periodic_faces",
+                "max_cell_faces", "fused", "centre_form")
+        return dict(zip(keys, [int(x) for x in c]))
+
     def close(self):
         if self.h:
             self.lib.teal_gpu_destroy(self.h)

@@ -35,6 +35,10 @@ struct DevData {
     int2 *d_bflr = nullptr;
     S *d_barea = nullptr, *d_bbasis = nullptr, *d_boffl = nullptr, *d_boffr = nullptr,
       *d_bcorr = nullptr;  // face geometry in block-list order (stride num_block_faces)
+    S *d_bfcen = nullptr;  // centre form: face centres in block-list order
+    S *d_ccen = nullptr;   // centre form: cell centres [num_cells][3], device rows
+    int *d_bcrow = nullptr;  // centre form: right-cell centre row per list entry
+    bool centre_form = false;
     int num_block_faces = 0;
     int max_block_faces = 0;
     int fused = 0;  // 1: residual passes use k_flux_block (its face lists fit in LDS and the
@@ -44,7 +48,8 @@ struct DevData {
     int build_blocks(const std::vector<int> &inc, int ni, const std::vector<int2> &flr,
                      const std::vector<double> &area, const std::vector<double> &basis,
                      const std::vector<double> &offl, const std::vector<double> &offr,
-                     const std::vector<double> &corr);
+                     const std::vector<double> &corr, const std::vector<double> &fcen,
+                     const std::vector<int> &crow, const std::vector<double> &ccen);
     void release();
     fv::Geom<S> geom() const;
     fv::BlockTopo block_topo() const;

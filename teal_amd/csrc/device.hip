@@ -342,11 +342,20 @@ int DevData<S>::build(const Mesh *m, TealGpu *gpu)
     std::vector<double> area(nf), basis((size_t)9 * nf), offl((size_t)3 * nf), offr((size_t)3 * nf),
         corr((size_t)4 * nf);
     std::vector<int2> flr(nf);
+    std::vector<int> crow(nf);
+    std::vector<double> fcen((size_t)3 * nf), ccen((size_t)3 * nc);
+    for (int d = 0; d < nc; d++) {
+        const double *x = &m->cells.center[cperm[d]].x;
+        for (int k = 0; k < 3; k++) ccen[((size_t)d * 3) + k] = x[k];
+    }
     for (int d = 0; d < nf; d++) {
         int f = fperm[d];
         int r = m->faces.cell_idx[f].right;
         flr[d].x = cinv[m->faces.cell_idx[f].left];
         flr[d].y = (f >= nfi && f < nfb) ? ~r : resolve(r);
+        crow[d] = cinv[r];  // ghosts keep their row; a periodic face's own ghost, not its source
+        const double *fc = &m->faces.center[f].x;
+        for (int k = 0; k < 3; k++) fcen[((size_t)k * nf) + d] = fc[k];
         area[d] = m->faces.area[f];
         const double *b = &m->faces.basis[f].x.x;
         for (int k = 0; k < 9; k++) basis[((size_t)k * nf) + d] = b[k];
@@ -394,7 +403,39 @@ int DevData<S>::build(const Mesh *m, TealGpu *gpu)
     HIPCALL(hipMemset(d_F, 0, sizeof(S) * 10 * (size_t)(nf + 1)));
     HIPCALL(hipMalloc((void **)&d_eps2, sizeof(S) * (size_t)ni));
     HIPCALL(hipMemset(d_eps2, 0, sizeof(S) * (size_t)ni));
-    return build_blocks(inc, ni, flr, area, basis, offl, offr, corr);
+    return build_blocks(inc, ni, flr, area, basis, offl, offr, corr, fcen, crow, ccen);
+}
+
+// true when the face offsets and corrections equal, bit for bit, their recomputation from
+// the face and cell centres with the expressions of src/mesh/generate.c:754-777
+static bool centres_reproduce(const std::vector<int2> &flr, const std::vector<int> &crow,
+                              const std::vector<double> &fcen, const std::vector<double> &ccen,
+                              const std::vector<double> &offl, const std::vector<double> &offr,
+                              const std::vector<double> &corr)
+{
+    const size_t nf = flr.size();
+    for (size_t f = 0; f < nf; f++) {
+        const double *cl = &ccen[(size_t)flr[f].x * 3];
+        const double *cr = &ccen[(size_t)crow[f] * 3];
+        double d[3], fc[3];
+        for (int k = 0; k < 3; k++) {
+            fc[k] = fcen[(k * nf) + f];
+            d[k] = cr[k] - cl[k];
+            if (fc[k] - cl[k] != offl[(k * nf) + f] || fc[k] - cr[k] != offr[(k * nf) + f]) {
+                return false;
+            }
+        }
+        double norm = sqrt((d[0] * d[0]) + (d[1] * d[1]) + (d[2] * d[2]));
+        if (norm != corr[(3 * nf) + f]) {
+            return false;
+        }
+        for (int k = 0; k < 3; k++) {
+            if (d[k] / norm != corr[(k * nf) + f]) {
+                return false;
+            }
+        }
+    }
+    return true;
 }
 
 // Face lists of the fused stage kernel: block b owns device cells [b*BLOCK, (b+1)*BLOCK); its
@@ -405,7 +446,8 @@ template <typename S>
 int DevData<S>::build_blocks(const std::vector<int> &inc, int ni, const std::vector<int2> &flr,
                              const std::vector<double> &area, const std::vector<double> &basis,
                              const std::vector<double> &offl, const std::vector<double> &offr,
-                             const std::vector<double> &corr)
+                             const std::vector<double> &corr, const std::vector<double> &fcen,
+                             const std::vector<int> &crow, const std::vector<double> &ccen)
 {
     const int nf = (int)area.size();
     const int nblk = (ni + BLOCK - 1) / BLOCK;
@@ -443,13 +485,30 @@ int DevData<S>::build_blocks(const std::vector<int> &inc, int ni, const std::vec
     const int nj = (int)list.size();
     const size_t cap = (size_t)std::max(nj, 1);
     num_block_faces = nj;
+    // FP64: the centre form (48 fewer bytes per face) where it reproduces the stored geometry;
+    // FP32 keeps the offsets rounded from FP64 (a difference of FP32 centres loses digits)
+    centre_form = sizeof(S) == 8 && centres_reproduce(flr, crow, fcen, ccen, offl, offr, corr);
     std::vector<int2> lflr(cap);
-    std::vector<double> larea(cap), lbasis(9 * cap), loffl(3 * cap), loffr(3 * cap), lcorr(4 * cap);
+    std::vector<int> lcrow(cap);
+    std::vector<double> larea(cap), lbasis(9 * cap), loffl, loffr, lcorr, lfcen;
+    if (centre_form) {
+        lfcen.assign(3 * cap, 0.0);
+    }
+    else {
+        loffl.assign(3 * cap, 0.0);
+        loffr.assign(3 * cap, 0.0);
+        lcorr.assign(4 * cap, 0.0);
+    }
     for (int j = 0; j < nj; j++) {
         int f = list[j];
         lflr[j] = flr[f];
+        lcrow[j] = crow[f];
         larea[j] = area[f];
         for (int k = 0; k < 9; k++) lbasis[((size_t)k * nj) + j] = basis[((size_t)k * nf) + f];
+        if (centre_form) {
+            for (int k = 0; k < 3; k++) lfcen[((size_t)k * nj) + j] = fcen[((size_t)k * nf) + f];
+            continue;
+        }
         for (int k = 0; k < 3; k++) {
             loffl[((size_t)k * nj) + j] = offl[((size_t)k * nf) + f];
             loffr[((size_t)k * nj) + j] = offr[((size_t)k * nf) + f];
@@ -457,8 +516,15 @@ int DevData<S>::build_blocks(const std::vector<int> &inc, int ni, const std::vec
         for (int k = 0; k < 4; k++) lcorr[((size_t)k * nj) + j] = corr[((size_t)k * nf) + f];
     }
     if (upload(&d_boff, off) || upload(&d_bslot, slot) || upload(&d_bflr, lflr) ||
-        upload_conv(&d_barea, larea) || upload_conv(&d_bbasis, lbasis) ||
-        upload_conv(&d_boffl, loffl) || upload_conv(&d_boffr, loffr) || upload_conv(&d_bcorr, lcorr)) {
+        upload_conv(&d_barea, larea) || upload_conv(&d_bbasis, lbasis)) {
+        return -1;
+    }
+    if (centre_form) {
+        if (upload(&d_bcrow, lcrow) || upload_conv(&d_bfcen, lfcen) || upload_conv(&d_ccen, ccen)) {
+            return -1;
+        }
+    }
+    else if (upload_conv(&d_boffl, loffl) || upload_conv(&d_boffr, loffr) || upload_conv(&d_bcorr, lcorr)) {
         return -1;
     }
     return 0;
@@ -473,7 +539,8 @@ BlockTopo DevData<S>::block_topo() const
 template <typename S>
 Geom<S> DevData<S>::block_geom() const
 {
-    return Geom<S>{d_vol, d_proj, d_eps2, d_w, d_o, d_barea, d_bbasis, d_boffl, d_boffr, d_bcorr};
+    return Geom<S>{d_vol,    d_proj,   d_eps2,   d_w,      d_o,    d_barea,
+                   d_bbasis, d_boffl,  d_boffr,  d_bcorr,  d_bfcen, d_ccen, d_bcrow};
 }
 
 template <typename S>
@@ -482,7 +549,7 @@ void DevData<S>::release()
     void *ptrs[] = {d_inc, d_nbr, d_flr, d_w, d_o, d_vol, d_proj, d_area, d_basis, d_offl,
                     d_offr, d_corr, d_vol64, d_bleft, d_bright, d_bface, d_bkind, d_bctx,
                     d_P, d_P2, d_GP, d_F, d_eps2, d_boff, d_bslot, d_bflr, d_barea, d_bbasis,
-                    d_boffl, d_boffr, d_bcorr};
+                    d_boffl, d_boffr, d_bcorr, d_bfcen, d_bcrow, d_ccen};
     for (void *p : ptrs) {
         if (p) {
             (void)hipFree(p);
@@ -493,7 +560,8 @@ void DevData<S>::release()
 template <typename S>
 Geom<S> DevData<S>::geom() const
 {
-    return Geom<S>{d_vol, d_proj, d_eps2, d_w, d_o, d_area, d_basis, d_offl, d_offr, d_corr};
+    return Geom<S>{d_vol,   d_proj, d_eps2, d_w,     d_o,     d_area, d_basis,
+                   d_offl,  d_offr, d_corr, nullptr, nullptr, nullptr};
 }
 
 template struct DevData<double>;
@@ -1226,6 +1294,10 @@ extern "C" int teal_gpu_info(TealGpu *g, long *counts)
     counts[3] = g->nfi;
     counts[4] = g->nfb;
     counts[5] = W;
+    const bool fused = (g->precision == TEAL_FP64) ? g->d64.fused : g->d32.fused;
+    const bool centre = (g->precision == TEAL_FP64) ? g->d64.centre_form : g->d32.centre_form;
+    counts[6] = fused;
+    counts[7] = fused && centre;
     return 0;
 }
 

@@ -75,6 +75,13 @@ struct Geom {
     const S *__restrict__ offl;   // [3][nf]
     const S *__restrict__ offr;   // [3][nf]
     const S *__restrict__ corr;   // [4][nf] unit xyz, norm
+    // centre form (offl, offr, corr null): the face offsets and corrections are recomputed
+    // from the face centre and the cell centres with the expressions that produced them
+    // (src/mesh/generate.c:754-777), so they are bit-identical
+    const S *__restrict__ fcen;   // [3][nf] face centres
+    const S *__restrict__ ccen;   // [num_cells][3] cell centres (device rows)
+    const int *__restrict__ crow; // [nf] centre row of the right cell (the ghost row of a
+                                  // periodic face, whose flr.y is its source cell)
 };
 
 struct Phys {
@@ -271,6 +278,26 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
     const int l = lr.x;
     const bool bnd = lr.y < 0;
     const int r = bnd ? ~lr.y : lr.y;
+    // face -> cell offsets and the centre correction (generate.c:754-777)
+    V3<S> ol, orr, cu;
+    S cn;
+    if (g.offl) {
+        ol = v3(g.offl[(0 * nf) + f], g.offl[(1 * nf) + f], g.offl[(2 * nf) + f]);
+        orr = v3(g.offr[(0 * nf) + f], g.offr[(1 * nf) + f], g.offr[(2 * nf) + f]);
+        cu = v3(g.corr[(0 * nf) + f], g.corr[(1 * nf) + f], g.corr[(2 * nf) + f]);
+        cn = g.corr[(3 * nf) + f];
+    }
+    else {
+        const long rr = g.crow[f];
+        V3<S> fc = v3(g.fcen[(0 * nf) + f], g.fcen[(1 * nf) + f], g.fcen[(2 * nf) + f]);
+        V3<S> cl = v3(g.ccen[((long)l * 3) + 0], g.ccen[((long)l * 3) + 1], g.ccen[((long)l * 3) + 2]);
+        V3<S> cr = v3(g.ccen[(rr * 3) + 0], g.ccen[(rr * 3) + 1], g.ccen[(rr * 3) + 2]);
+        ol = v3(fc.x - cl.x, fc.y - cl.y, fc.z - cl.z);
+        orr = v3(fc.x - cr.x, fc.y - cr.y, fc.z - cr.z);
+        V3<S> dl = v3(cr.x - cl.x, cr.y - cl.y, cr.z - cl.z);
+        cn = sqrt(dot(dl, dl));
+        cu = v3(dl.x / cn, dl.y / cn, dl.z / cn);
+    }
     S pl[5], pr[5];
     load_rec(pl, P, l, 5);
     load_rec(pr, P, r, 5);
@@ -297,8 +324,6 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
                            (gl[(v * 3) + 2] + gr[(v * 3) + 2]) / 2);
             }
         }
-        V3<S> cu = v3(g.corr[(0 * nf) + f], g.corr[(1 * nf) + f], g.corr[(2 * nf) + f]);
-        S cn = g.corr[(3 * nf) + f];
         for (int v = 0; v < 5; v++) {  // correct, derivative.c:81-89
             S proj = dot(gm[v], cu);
             S adjust = proj - ((pr[v] - pl[v]) / cn);
@@ -315,7 +340,6 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
     if (FLUX != FLUX_NONE) {
         S ql[5], qr[5];
         if (ORDER == 2) {  // reconstruct with the limited gradient G*psi (derivative.c:158-164)
-            V3<S> ol = v3(g.offl[(0 * nf) + f], g.offl[(1 * nf) + f], g.offl[(2 * nf) + f]);
             for (int v = 0; v < 5; v++) {
                 S ps = gl[15 + v];
                 V3<S> gv = v3(gl[(v * 3) + 0] * ps, gl[(v * 3) + 1] * ps, gl[(v * 3) + 2] * ps);
@@ -325,7 +349,6 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
                 for (int v = 0; v < 5; v++) qr[v] = pr[v];
             }
             else {
-                V3<S> orr = v3(g.offr[(0 * nf) + f], g.offr[(1 * nf) + f], g.offr[(2 * nf) + f]);
                 for (int v = 0; v < 5; v++) {
                     S ps = gr[15 + v];
                     V3<S> gv = v3(gr[(v * 3) + 0] * ps, gr[(v * 3) + 1] * ps, gr[(v * 3) + 2] * ps);

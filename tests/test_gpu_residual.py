@@ -83,3 +83,15 @@ def test_lserk_steps_bitwise(name, case, kind, path):
     # by the next exchange, so they are not part of the state)
     nb = arrays["off_boundary"]
     np.testing.assert_array_equal(Pg[:nb], po.reshape(-1, 5)[:nb])
+
+
+def test_tgv_layout_uses_fused_centre_form():
+    """The TGV mesh's stored face offsets and corrections are reproduced bit for bit from the
+    centres, so the FP64 stage kernel reads the centre form; the mixed path keeps the arrays."""
+    pm = teal_amd.mesh_box((16, 16, 16), (0, 0, 0), (2 * np.pi,) * 3, (1, 1, 1))
+    g64 = teal_amd.Gpu(pm, teal_amd.FP64)
+    g32 = teal_amd.Gpu(pm, teal_amd.MIXED)
+    assert g64.info()["fused"] == 1 and g64.info()["centre_form"] == 1
+    assert g32.info()["fused"] == 1 and g32.info()["centre_form"] == 0
+    g64.close()
+    g32.close()
