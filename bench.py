@@ -246,6 +246,72 @@ def tet_workload(args, n=96, seed=2024):
         "runs": out}), flush=True)
 
 
+def channel_workload(args, n=64, seed=5):
+    """Navier-Stokes implicit Euler with Jacobian-free Newton-GMRES (NewtonKrylov {0.1, 0.3,
+    30}) in a channel [0,2] x [-1,1] x [0,1]: periodic in x and z, no-slip walls in y with the
+    node levels stretched geometrically (ratio 1.05) toward both walls, parabolic profile plus
+    1% seeded noise, Re = 100 on the half height, M = 0.2, HLLC + central viscous, BJ.
+    `args.steps` implicit steps at CFL `args.courant`; FP64 and mixed."""
+    lib = teal_amd.lib()
+    mesh = lib.mesh_create(teal_amd.Vector(0, -1, 0), teal_amd.Vector(2, 1, 1),
+                           teal_amd.Triple(n, n, n), teal_amd.Triple(1, 0, 1))
+    half = n // 2
+    widths = 1.05 ** np.arange(half)
+    edges = np.concatenate([[0.0], np.cumsum(widths)]) / widths.sum()
+    levels = np.concatenate([-1 + edges, (1 - edges[::-1])[1:]])
+    coord = _abi.mesh_arrays(mesh)["coord"]
+    coord[:, 1] = levels[np.rint((coord[:, 1] + 1) / (2.0 / n)).astype(int)]
+    lib.mesh_generate(mesh)
+    a = _abi.mesh_arrays(mesh)
+    nc, ni = a["num_cells"], a["num_inner"]
+    y = a["center"][:nc, 1]
+    P0 = np.zeros((nc, 5))
+    P0[:, 0] = 1.0
+    P0[:, 1] = (1 - y ** 2) * (1 + 0.01 * np.random.default_rng(seed).standard_normal(nc))
+    P0[:, 4] = 1.0 / (GAMMA * 0.2 ** 2)
+    P0[ni:] = 0
+    out = {}
+    for name, prec in (("fp64", teal_amd.FP64), ("mixed", teal_amd.MIXED)):
+        g = teal_amd.Gpu(mesh, prec)
+        g.configure(space_order=2, flux="hllc", viscous=1, limiter="minmod", navierstokes=1,
+                    gamma=GAMMA, mu=0.01, prandtl=PR,
+                    bcs={"bottom": ("wall", None), "top": ("wall", None)})
+        g.upload(P0)
+        g.implicit_euler(0.0, args.courant, 0.1, 0.3, 30)  # warm-up step (allocations)
+        g.upload(P0)
+        t, newton, krylov, res0 = 0.0, 0, 0, None
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            _, t, res = g.implicit_euler(t, args.courant, 0.1, 0.3, 30)
+            st = g.krylov_stats()
+            newton += st["newton"]
+            krylov += st["krylov"]
+            res0 = res if res0 is None else res0
+            print(f"#   {name} step {i}: {st['newton']} Newton / {st['krylov']} GMRES iterations, "
+                  f"residual {max(res):.3g}", file=sys.stderr, flush=True)
+        secs = time.perf_counter() - t0
+        g.close()
+        out[name] = {"ms_per_step": 1e3 * secs / args.steps, "newton_iterations": newton,
+                     "gmres_iterations": krylov, "matvecs_per_s": krylov / secs,
+                     "residual_first": float(max(res0)), "residual_last": float(max(res)),
+                     "sim_time": t}
+        print(f"# channel {name}: {1e3 * secs / args.steps:.2f} ms/implicit step, {newton} Newton / "
+              f"{krylov} GMRES iterations in {args.steps} steps, {krylov / secs:.0f} matvecs/s, "
+              f"residual {max(res0):.3g} -> {max(res):.3g}", file=sys.stderr, flush=True)
+    mx = out["mixed"]
+    print(json.dumps({
+        "metric": "implicit Euler (Newton-GMRES) matvecs/s, NS channel 64^3",
+        "value": mx["matvecs_per_s"], "unit": "matvecs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": 1, "ms_per_step": mx["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 matvec residuals + f64 Krylov vectors / Gram-Schmidt (mixed)",
+        "data": "synthetic",
+        "config": {"workload": f"NS channel {n}^3, walls stretched 1.05, Re 100, M 0.2, implicit "
+                               f"Euler CFL {args.courant}, NewtonKrylov (0.1, 0.3, 30)",
+                   "cells": ni},
+        "runs": out}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -257,9 +323,12 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE/"
                          "WRITE_SIZE against a kernel of known traffic)")
-    ap.add_argument("--workload", choices=["tgv", "tet"], default="tgv",
+    ap.add_argument("--workload", choices=["tgv", "tet", "channel"], default="tgv",
                     help="tgv: the NS Taylor-Green vortex (the headline metric); tet: Euler "
-                         "Mach-2 bump on the jittered tetrahedral mesh (Roe and AUSMD)")
+                         "Mach-2 bump on the jittered tetrahedral mesh (Roe and AUSMD); channel: "
+                         "NS implicit Euler (Newton-GMRES) in a 64^3 stretched-wall channel")
+    ap.add_argument("--courant", type=float, default=50.0, help="channel workload CFL")
+    ap.add_argument("--channel-n", type=int, default=64, help="channel workload cells per side")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
@@ -279,6 +348,9 @@ def main():
 
     if args.workload == "tet":
         tet_workload(args)
+        return
+    if args.workload == "channel":
+        channel_workload(args, n=args.channel_n)
         return
     n = args.n
     mesh = teal_amd.mesh_box((n, n, n), (0.0, 0.0, 0.0), (2 * np.pi,) * 3, (1, 1, 1))

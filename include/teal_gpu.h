@@ -105,6 +105,11 @@ double teal_gpu_implicit_euler(TealGpu *gpu, double *time, double *residual, dou
 enum TealReduction { TEAL_REDUCE_TREE = 0, TEAL_REDUCE_SERIAL = 1 };
 int teal_gpu_set_reductions(TealGpu *gpu, int mode);
 
+// Iteration counts of the last teal_gpu_implicit_euler call: stats[0] Newton iterations,
+// stats[1] GMRES iterations (one matvec, i.e. one residual evaluation, each).  No counterpart
+// in the reference, which does not report them.
+int teal_gpu_krylov_stats(TealGpu *gpu, long *stats);
+
 // `steps` lserk steps with no host synchronization between them (the inner loop of
 // src/simulation/run.c:68-102 without output); the final time is returned in *time.
 int teal_gpu_run_lserk(TealGpu *gpu, int steps, double *time, double courant, int time_order,

@@ -83,6 +83,7 @@ def lib():
         _sig(l, "teal_gpu_implicit_euler", D, VP, DP, DP, D, D, D, D, C.c_int)
         _sig(l, "teal_gpu_run_lserk", C.c_int, VP, C.c_int, DP, D, C.c_int, C.c_int)
         _sig(l, "teal_gpu_set_reductions", C.c_int, VP, C.c_int)
+        _sig(l, "teal_gpu_krylov_stats", C.c_int, VP, LP)
         _sig(l, "teal_gpu_timing", C.c_int, VP, C.c_int)
         _sig(l, "teal_gpu_timing_ms", D, VP, C.c_int, IP)
         _sig(l, "teal_gpu_info", C.c_int, VP, LP)
@@ -179,6 +180,11 @@ class Gpu:
         if np.isnan(step0):
             raise RuntimeError(f"implicit_euler failed: {last_error()}")
         return step0, t.value, np.array(res[:])
+
+    def krylov_stats(self):
+        st = (C.c_long * 2)()
+        check(self.lib.teal_gpu_krylov_stats(self.h, st), "krylov_stats")
+        return {"newton": int(st[0]), "krylov": int(st[1])}
 
     def set_reductions(self, mode):
         check(self.lib.teal_gpu_set_reductions(self.h, mode), "set_reductions")

@@ -104,6 +104,10 @@ struct TealGpu {
     DevData<double> d64;
     DevData<float> d32;
     TealGpuConfig cfg{};
+    std::vector<int> cfg_bc;        // the configuration's per-entity arrays, kept for mirrors
+    std::vector<double> cfg_ctx;    // built later (the FP64 mirror of a mixed TealGpu)
+    std::vector<double> cfg_eps2;
+    int lds_max = 0;                // dynamic LDS a workgroup may use
     int configured = 0;
     int u0_valid = 0;  // d_U0 holds the conserved state of the current step start
     int dt_valid = 0;  // the scalars hold the dt minimum of that state
@@ -134,3 +138,6 @@ void teal_krylov_release(TealGpu *g);
 // residual = sqrt(sum V R^2 / sum V) of a device derivative [5][ni] (src/equations/residual.c)
 int teal_residual_norm(TealGpu *g, const double *d_derivative, double *residual);
 void teal_device_numbering(TealGpu *g, const Mesh *m);
+// FP64 device arrays and configuration on a mixed TealGpu (its implicit solver evaluates the
+// nonlinear residual in FP64); no-op on the FP64 path or when already built
+int teal_fp64_mirror(TealGpu *g);

@@ -604,8 +604,9 @@ extern "C" TealGpu *teal_gpu_create(const Mesh *mesh, int precision)
             lds_max = 0;
         }
         int want = !(split && atoi(split) != 0);
-        g->d64.fused = want && g->d64.block_lds() > 0 && g->d64.block_lds() <= (size_t)lds_max;
-        g->d32.fused = want && g->d32.block_lds() > 0 && g->d32.block_lds() <= (size_t)lds_max;
+        g->lds_max = want ? lds_max : 0;
+        g->d64.fused = g->d64.block_lds() > 0 && g->d64.block_lds() <= (size_t)g->lds_max;
+        g->d32.fused = g->d32.block_lds() > 0 && g->d32.block_lds() <= (size_t)g->lds_max;
     }
     if (rc == 0 && upload(&g->d_cell_perm, g->cell_perm)) {
         rc = -1;
@@ -740,9 +741,38 @@ extern "C" int teal_gpu_configure(TealGpu *g, const TealGpuConfig *cfg)
     g->cfg.bc = nullptr;
     g->cfg.bc_context = nullptr;
     g->cfg.eps2 = nullptr;
+    int nent = cfg->num_boundary > 0 ? cfg->num_boundary : 0;
+    g->cfg_bc.assign(cfg->bc ? cfg->bc : nullptr, cfg->bc ? cfg->bc + nent : nullptr);
+    g->cfg_ctx.assign(cfg->bc_context ? cfg->bc_context : nullptr,
+                      cfg->bc_context ? cfg->bc_context + (5 * nent) : nullptr);
+    g->cfg_eps2.assign(cfg->eps2 ? cfg->eps2 : nullptr, cfg->eps2 ? cfg->eps2 + g->ni : nullptr);
     g->configured = 1;
     g->u0_valid = 0;
-    return (g->precision == TEAL_FP64) ? configure_impl(g, g->d64, cfg) : configure_impl(g, g->d32, cfg);
+    if (g->precision == TEAL_FP64) {
+        return configure_impl(g, g->d64, cfg);
+    }
+    if (g->d64.d_P && configure_impl(g, g->d64, cfg)) {  // the FP64 mirror, when built
+        return -1;
+    }
+    return configure_impl(g, g->d32, cfg);
+}
+
+int teal_fp64_mirror(TealGpu *g)
+{
+    if (g->precision == TEAL_FP64 || g->d64.d_P) {
+        return 0;
+    }
+    if (g->d64.build(g->mesh, g)) {
+        g->d64.release();
+        g->d64 = DevData<double>();
+        return -1;
+    }
+    g->d64.fused = g->d64.block_lds() > 0 && g->d64.block_lds() <= (size_t)g->lds_max;
+    TealGpuConfig c = g->cfg;
+    c.bc = g->cfg_bc.empty() ? nullptr : g->cfg_bc.data();
+    c.bc_context = g->cfg_ctx.empty() ? nullptr : g->cfg_ctx.data();
+    c.eps2 = g->cfg_eps2.empty() ? nullptr : g->cfg_eps2.data();
+    return configure_impl(g, g->d64, &c);
 }
 
 // ------------------------------------------------------------------ kernel launches

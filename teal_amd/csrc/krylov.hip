@@ -33,6 +33,13 @@ using namespace fv;
         }                                                                                 \
     } while (0)
 
+// Finite-difference scale of the matvec when the residual is evaluated in FP32.  The
+// reference scales the perturbation by the global norm |U|, which the energy dominates, so the
+// momentum components see a perturbation (|E|/|rho u|) times larger than this factor.
+#ifndef TEAL_MIXED_FD_RELATIVE
+#define TEAL_MIXED_FD_RELATIVE 3.4526698300124393e-4
+#endif
+
 namespace {
 
 constexpr int RT = 256;        // threads of the vector kernels
@@ -274,7 +281,7 @@ __global__ __launch_bounds__(RT) void k_increment(double *__restrict__ inc, cons
     }
 }
 
-// Newton update (advance.c:167-172): U += increment, P = cons2prim(U)
+// Newton update (advance.c:167-172): U += increment (when given), P = cons2prim(U)
 template <typename S>
 __global__ __launch_bounds__(RT) void k_newton_update(double *__restrict__ U, const double *__restrict__ inc,
                                                       int ni, double gamma, S *__restrict__ P)
@@ -283,8 +290,10 @@ __global__ __launch_bounds__(RT) void k_newton_update(double *__restrict__ U, co
         double u[5], p[5];
         for (int k = 0; k < 5; k++) {
             long e = ((long)k * ni) + c;
-            u[k] = U[e] + inc[e];
-            U[e] = u[k];
+            u[k] = inc ? U[e] + inc[e] : U[e];
+            if (inc) {
+                U[e] = u[k];
+            }
         }
         cons2prim<double>(p, u, gamma);
         for (int k = 0; k < 5; k++) P[((long)c * 5) + k] = (S)p[k];
@@ -322,7 +331,8 @@ struct Krylov {
     double *V = nullptr;      // (dim + 1) basis vectors
     double *w = nullptr;      // matvec result / Gram-Schmidt vector
     double *U = nullptr;      // conserved state
-    double *D = nullptr;      // derivative at U
+    double *D = nullptr;      // derivative at U (the nonlinear residual, FP64)
+    double *Dm = nullptr;     // mixed: FP32-path derivative at U, the matvec's base point
     double *Rn = nullptr;     // Newton residual
     double *inc = nullptr;    // Newton increment
     double *coef = nullptr;   // back-substitution coefficients (dim)
@@ -330,6 +340,7 @@ struct Krylov {
     double *scal = nullptr;   // [0..dim]: Hessenberg column, [dim+1]: eps, [dim+2..]: norms
     double *h_scal = nullptr; // pinned copy of scal
     int *dev_of_ref = nullptr; // serial reductions: device row of each inner cell (reference order)
+    long newton = 0, krylov = 0;  // iterations of the last implicit step (Newton, GMRES)
 };
 
 void teal_krylov_release(TealGpu *g)
@@ -338,8 +349,8 @@ void teal_krylov_release(TealGpu *g)
         return;
     }
     Krylov *k = g->krylov;
-    void *ptrs[] = {k->V, k->w, k->U, k->D, k->Rn, k->inc, k->coef, k->part[0], k->part[1], k->scal,
-                    k->dev_of_ref};
+    void *ptrs[] = {k->V, k->w, k->U, k->D, k->Dm, k->Rn, k->inc, k->coef, k->part[0], k->part[1],
+                    k->scal, k->dev_of_ref};
     for (void *p : ptrs) {
         if (p) {
             (void)hipFree(p);
@@ -368,6 +379,7 @@ static int krylov_reserve(TealGpu *g, int dim)
     HIPCALL(hipMalloc((void **)&k->w, vec));
     HIPCALL(hipMalloc((void **)&k->U, vec));
     HIPCALL(hipMalloc((void **)&k->D, vec));
+    HIPCALL(hipMalloc((void **)&k->Dm, vec));
     HIPCALL(hipMalloc((void **)&k->Rn, vec));
     HIPCALL(hipMalloc((void **)&k->inc, vec));
     HIPCALL(hipMalloc((void **)&k->coef, sizeof(double) * (size_t)dim));
@@ -421,8 +433,8 @@ int derivative_into(TealGpu *g, DevData<S> &d, Krylov *k, double *out)
 
 // gmres (src/simulation/gmres.c:52-149); the increment lands in k->inc
 template <typename S>
-int gmres(TealGpu *g, DevData<S> &d, Krylov *k, double step, double norm, double fd_scale,
-          double tol, int dim)
+int gmres(TealGpu *g, DevData<S> &d, Krylov *k, const double *Dbase, double step, double norm,
+          double fd_scale, double tol, int dim)
 {
     const long n = k->n;
     const int nb = parts_of(n);
@@ -460,7 +472,7 @@ int gmres(TealGpu *g, DevData<S> &d, Krylov *k, double step, double norm, double
         HIPCALL(hipGetLastError());
         if (teal_residual_pass(g, d, 0, 0.0, 0)) return -1;
         hipLaunchKernelGGL(k_matvec_finish, dim3(nb), dim3(RT), 0, g->stream, (const double *)v,
-                           (const double *)g->d_R, (const double *)k->D, step, (const double *)eps_dev,
+                           (const double *)g->d_R, Dbase, step, (const double *)eps_dev,
                            k->w, n);
         // modified Gram-Schmidt (gmres.c:92-100): hess[i][iter] lands in scal[i]
         if (g->serial_reductions) {
@@ -509,6 +521,7 @@ int gmres(TealGpu *g, DevData<S> &d, Krylov *k, double step, double norm, double
         H(iter, iter) = rot;
         rhs[iter + 1] = -sine[iter] * rhs[iter];
         rhs[iter] = cosine[iter] * rhs[iter];
+        k->krylov += 1;
         if (fabs(rhs[iter + 1]) < tol_norm) {
             iter += 1;
             break;
@@ -532,43 +545,64 @@ int gmres(TealGpu *g, DevData<S> &d, Krylov *k, double step, double norm, double
     return 0;
 }
 
-// implicit_euler (src/simulation/advance.c:119-195) on the resident state
+// implicit_euler (src/simulation/advance.c:119-195) on the resident state.  dm is the device
+// data of the configured path; the matvec's residuals run on it.  The nonlinear (Newton)
+// residual always runs in FP64: on the mixed path the FP32 residual's rounding noise would
+// set a floor under the Newton residual, so a mixed TealGpu keeps an FP64 mirror for it and
+// the matvec differences FP32 residuals about their own FP32 base point.
 template <typename S>
-int implicit_step(TealGpu *g, DevData<S> &d, double *time, double *residual, double max_step,
+int implicit_step(TealGpu *g, DevData<S> &dm, double *time, double *residual, double max_step,
                   double courant, double newton_tol, double krylov_tol, int dim, double *step0_out)
 {
-    if (krylov_reserve(g, dim)) return -1;
+    const bool mixed = g->precision == TEAL_MIXED;
+    if (krylov_reserve(g, dim) || teal_fp64_mirror(g)) return -1;
+    DevData<double> &dn = g->d64;
     Krylov *k = g->krylov;
     const long n = k->n;
     const int nb = parts_of(n);
     const int cg = cells_grid(g->ni);
+    const double gamma = g->cfg.gamma;
     double dt = 0;
     // dt minimum, and U0 = prim2cons(P) unless U0 already holds the state (mixed: its FP64 copy)
-    if (teal_timestep_pass(g, d, !g->u0_valid, &dt)) return -1;
+    if (teal_timestep_pass(g, dm, !g->u0_valid, &dt)) return -1;
     const double step0 = courant * dt;
     const double step = fmin(step0, max_step);
     HIPCALL(hipMemcpyAsync(k->U, g->d_U0, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice,
                            g->stream));
+    // residuals of the state U: FP64 (Newton) and, on the mixed path, FP32 (matvec base)
+    auto residuals = [&]() -> int {
+        if (mixed) {
+            hipLaunchKernelGGL((k_newton_update<double>), dim3(cg), dim3(RT), 0, g->stream, k->U,
+                               (const double *)nullptr, g->ni, gamma, dn.d_P);
+            HIPCALL(hipGetLastError());
+            if (derivative_into(g, dm, k, k->Dm)) return -1;
+        }
+        return derivative_into(g, dn, k, k->D);
+    };
+    const double *Dbase = mixed ? k->Dm : k->D;
     *time += step;
-    if (derivative_into(g, d, k, k->D)) return -1;
+    if (residuals()) return -1;
     hipLaunchKernelGGL(k_newton_residual, dim3(nb), dim3(RT), 0, g->stream, k->Rn, (const double *)nullptr,
                        (const double *)nullptr, (const double *)k->D, step, n);
     const int slot = dim + 2;
     if (two_norms(g, k, k->Rn, k->U, slot)) return -1;
     double norm = k->h_scal[slot];
     double cons_norm = k->h_scal[slot + 1];
-    // the finite-difference step: the reference's 1e-6 on the FP64 path; the square root of
-    // the FP32 epsilon where the residual is evaluated in FP32
-    const double fd_relative = (g->precision == TEAL_FP64) ? 1e-6 : 3.4526698300124393e-4;
+    // the finite-difference step: the reference's 1e-6 on the FP64 path; on the FP32 path
+    // TEAL_MIXED_FD_RELATIVE (see its definition)
+    const double fd_relative = mixed ? TEAL_MIXED_FD_RELATIVE : 1e-6;
     double fd_scale = fd_relative * fmax(1, cons_norm) / fmax(1, norm);
     const double tol_norm = newton_tol * norm;
-    const long max_iter = 128L * (g->ni + 1);
+    const long max_iter = 128L * (g->ni + 1);  // advance.c:162
+    k->newton = 0;
+    k->krylov = 0;
     for (long iter = 0; iter < max_iter && norm > tol_norm; iter++) {
-        if (gmres(g, d, k, step, norm, fd_scale, krylov_tol, dim)) return -1;
+        k->newton += 1;
+        if (gmres(g, dm, k, Dbase, step, norm, fd_scale, krylov_tol, dim)) return -1;
         hipLaunchKernelGGL((k_newton_update<S>), dim3(cg), dim3(RT), 0, g->stream, k->U,
-                           (const double *)k->inc, g->ni, g->cfg.gamma, d.d_P);
+                           (const double *)k->inc, g->ni, gamma, dm.d_P);
         HIPCALL(hipGetLastError());
-        if (derivative_into(g, d, k, k->D)) return -1;
+        if (residuals()) return -1;
         hipLaunchKernelGGL(k_newton_residual, dim3(nb), dim3(RT), 0, g->stream, k->Rn,
                            (const double *)k->U, (const double *)g->d_U0, (const double *)k->D, step, n);
         if (two_norms(g, k, k->Rn, k->U, slot)) return -1;
@@ -577,7 +611,7 @@ int implicit_step(TealGpu *g, DevData<S> &d, double *time, double *residual, dou
         fd_scale = fd_relative * fmax(1, cons_norm) / fmax(1, norm);
     }
     if (g->serial_reductions) {
-        const double *vol64 = (g->precision == TEAL_FP64) ? g->d64.d_vol64 : g->d32.d_vol64;
+        const double *vol64 = g->d64.d_vol64;
         hipLaunchKernelGGL(k_serial_residual, dim3(1), dim3(RT), 0, g->stream, (const double *)k->D,
                            vol64, (const int *)k->dev_of_ref, g->ni, g->sum_volume, k->scal);
         HIPCALL(hipGetLastError());
@@ -588,7 +622,7 @@ int implicit_step(TealGpu *g, DevData<S> &d, double *time, double *residual, dou
     else if (teal_residual_norm(g, k->D, residual)) {
         return -1;
     }
-    if (g->precision == TEAL_MIXED) {  // carry the FP64 solution into the next step
+    if (mixed) {  // carry the FP64 solution into the next step
         HIPCALL(hipMemcpyAsync(g->d_U0, k->U, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice,
                                g->stream));
         g->u0_valid = 1;
@@ -602,6 +636,17 @@ int implicit_step(TealGpu *g, DevData<S> &d, double *time, double *residual, dou
 }
 
 }  // namespace
+
+extern "C" int teal_gpu_krylov_stats(TealGpu *g, long *stats)
+{
+    if (!g || !stats) {
+        teal_gpu_set_error("krylov_stats: null argument");
+        return -1;
+    }
+    stats[0] = g->krylov ? g->krylov->newton : 0;
+    stats[1] = g->krylov ? g->krylov->krylov : 0;
+    return 0;
+}
 
 extern "C" int teal_gpu_set_reductions(TealGpu *g, int mode)
 {

@@ -5,9 +5,10 @@ Every vector operation matches the reference element by element.  With serial re
 (the reference's summation order) the FP64 path is bit-for-bit.  With the default tree
 reductions the dot products round differently in the last bit, which the finite-difference
 matvec amplifies (an ulp in R(P1) - R divided by eps ~ 1e-6 |U|): the bound below is the
-measured spread with a margin.  The mixed path evaluates the matvec's residuals in FP32 with
-a finite-difference step of sqrt(FP32 epsilon); both paths stop the Newton iteration at the
-same relative tolerance, so they differ by about that tolerance times the step's change."""
+measured spread with a margin.  The mixed path evaluates the matvec's residuals in FP32 (about
+their own FP32 base point, with a finite-difference step of sqrt(FP32 epsilon)) and the
+Newton residual in FP64; both paths stop the Newton iteration at the same relative tolerance,
+so they differ by about that tolerance times the step's change."""
 import ctypes as C
 
 import numpy as np
