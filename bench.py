@@ -331,8 +331,8 @@ def main():
     ap.add_argument("--channel-n", type=int, default=64, help="channel workload cells per side")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
-              "(see the sharding design in DESIGN.md)", file=sys.stderr)
+        print("bench.py: this library drives one GPU per process; runs on several GPUs are "
+              "not part of it", file=sys.stderr)
         sys.exit(2)
     if args.copy_probe:
         import torch

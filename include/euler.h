@@ -36,7 +36,7 @@ Equations *euler_create(const Mesh *mesh); // src/euler.h:29
 EulerPrimitive euler_riemann(EulerPrimitive left, EulerPrimitive right, double gamma,
                              double loc);  // src/euler.h:32
 
-// Airfoil polar post-processing (src/euler.h:35): not part of this build (teal_error).
+// Airfoil polar post-processing (src/euler.h:35): not provided by this library (teal_error).
 void euler_polar(const Simulation *sim, const char *entity, EulerPrimitive reference,
                  double length, double time);
 

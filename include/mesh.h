@@ -85,7 +85,7 @@ typedef struct {
 // Cartesian hex mesh with optional per-axis periodicity (src/mesh.h:75, create.c:509-544).
 Mesh *mesh_create(Vector min_coord, Vector max_coord, Triple num_cells, Triple periodic);
 
-// Gmsh input (src/mesh.h:78): not part of this build; reports an error through teal_error.
+// Gmsh input (src/mesh.h:78): not provided by this library; reports an error through teal_error.
 Mesh *mesh_read(const char *fname);
 
 // Modify all node coordinates (src/mesh.h:81, src/mesh/modify.c:5-11).
@@ -104,7 +104,7 @@ void mesh_validate(const Mesh *mesh);
 // Print a mesh summary (src/mesh.h:93, src/mesh/summary.c:7-29).
 void mesh_summary(const Mesh *mesh);
 
-// VTKHDF output (src/mesh.h:96): not part of this build; reports an error.
+// VTKHDF output (src/mesh.h:96): not provided by this library; reports an error.
 void mesh_write(const Mesh *mesh, const char *name);
 
 // Release all mesh resources (src/mesh.h:99).

@@ -1,8 +1,8 @@
 // sync.h -- the process-group globals of the drop-in library.
 // Replaces the reference interface src/sync.h:5-9 for drivers that read `sync.rank` /
 // `sync.size` (e.g. output prefixes): the library drives one GPU from one process, so the
-// group is rank 0 of 1.  The MPI-typed collectives of the reference header are not part of
-// this build; the double-precision dot and norm are (src/core/sync.c:66-79, serial sums).
+// group is rank 0 of 1.  The MPI-typed collectives of the reference header are not provided by
+// this library; the double-precision dot and norm are (src/core/sync.c:66-79, serial sums).
 #pragma once
 
 #ifdef __cplusplus

@@ -499,7 +499,7 @@ void equations_write(const Equations *eqns, const char *name, double time, int i
     (void)eqns, (void)time;
     static int warned = 0;
     if (!warned) {
-        teal_print("\t (VTKHDF output is not part of this build: %s_%05d skipped)", name, index);
+        teal_print("\t (VTKHDF output is not provided by this library: %s_%05d skipped)", name, index);
         warned = 1;
     }
 }
@@ -507,7 +507,7 @@ void equations_write(const Equations *eqns, const char *name, double time, int i
 void equations_read(const Equations *eqns, const char *fname, double *time, int *index)
 {
     (void)eqns, (void)time, (void)index;
-    teal_error("restart input is not part of this build (%s)", fname);
+    teal_error("restart input is not provided by this library (%s)", fname);
 }
 
 void equations_destroy(Equations *eqns)

@@ -285,13 +285,13 @@ void mesh_destroy(Mesh *mesh)
 
 Mesh *mesh_read(const char *fname)
 {
-    teal_error("mesh_read is not part of this build (%s)", fname);
+    teal_error("mesh_read is not provided by this library (%s)", fname);
 }
 
 void mesh_write(const Mesh *mesh, const char *name)
 {
     (void)mesh;
-    teal_error("mesh_write is not part of this build (%s)", name);
+    teal_error("mesh_write is not provided by this library (%s)", name);
 }
 
 // ------------------------------------------------------------------ tetrahedral mesh

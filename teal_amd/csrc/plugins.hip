@@ -246,10 +246,10 @@ extern "C" Equations *navierstokes_create(const Mesh *mesh)
     return eqns;
 }
 
-// Airfoil polar post-processing is outside this build (src/euler/polar.c:22-83).
+// Airfoil polar post-processing is not provided by this library (src/euler/polar.c:22-83).
 extern "C" void euler_polar(const Simulation *sim, const char *entity, EulerPrimitive reference,
                             double length, double time)
 {
     (void)sim, (void)reference, (void)length, (void)time;
-    teal_error("euler_polar is not part of this build (%s)", entity);
+    teal_error("euler_polar is not provided by this library (%s)", entity);
 }
