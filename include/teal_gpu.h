@@ -19,6 +19,13 @@ extern "C" {
 
 typedef struct TealGpu TealGpu;
 
+// Disjoint union of generated meshes (no counterpart in the reference, which advances one mesh
+// per program): cells [inner | boundary ghosts | periodic ghosts] and faces [inner | boundary |
+// periodic], each group in instance order; entities [inner | boundary | periodic] likewise;
+// periodic neighbour tags made distinct per instance.  Geometry is copied, indices renumbered,
+// so each instance's part is bit-identical to the instance.  Free with mesh_destroy.
+Mesh *teal_mesh_union(const Mesh *const *meshes, int num);
+
 enum TealPrecision {
     TEAL_FP64 = 0,  // FP64 storage and arithmetic: bit-compatible with the reference
     TEAL_MIXED = 1, // FP32 states/gradients/geometry/flux math, compensated FP32 face
@@ -114,6 +121,38 @@ int teal_gpu_krylov_stats(TealGpu *gpu, long *stats);
 // src/simulation/run.c:68-102 without output); the final time is returned in *time.
 int teal_gpu_run_lserk(TealGpu *gpu, int steps, double *time, double courant, int time_order,
                        int num_stages);
+
+// Batches (no counterpart in the reference, which advances one mesh per program): many
+// independent generated meshes of any sizes advanced together, every kernel launch covering
+// all of them, each instance with its own time step, time and residual.  The handle works
+// on teal_mesh_union(meshes, num): teal_gpu_configure takes the union's boundary entities
+// (each mesh's in turn), teal_gpu_upload/download and the per-call routines the union's
+// layout.  teal_gpu_upload_batch / teal_gpu_download_batch take one primitive array per mesh
+// in its own layout.  teal_gpu_run_lserk_batch runs `steps` lserk steps on every instance
+// (time[i] in/out, residual[5 * i + v] of the last step when not null); the single-mesh
+// Advance methods and teal_gpu_implicit_euler refuse a batch handle.  teal_gpu_residual on a
+// batch writes 5 values per instance; teal_gpu_timestep returns the minimum over the union.
+TealGpu *teal_gpu_create_batch(const Mesh *const *meshes, int num, int precision);
+int teal_gpu_batch_size(TealGpu *gpu);
+const Mesh *teal_gpu_mesh(TealGpu *gpu);  // the mesh the handle works on (a batch: the union)
+int teal_gpu_upload_batch(TealGpu *gpu, const double *const *primitives);
+int teal_gpu_download_batch(TealGpu *gpu, double *const *primitives);
+int teal_gpu_run_lserk_batch(TealGpu *gpu, int steps, double *time, double *residual,
+                             double courant, int time_order, int num_stages);
+
+// Graph replay (no counterpart in the reference): with graphs enabled, teal_gpu_run_lserk
+// and teal_gpu_run_lserk_batch capture two explicit steps into a HIP graph on first use and
+// replay it, one launch per pair of steps; results are identical to eager launches.  The graph
+// is recaptured when the configuration or the coefficients change.
+int teal_gpu_set_graphs(TealGpu *gpu, int enable);
+
+// Launch-shape autotuner (no counterpart in the reference): times `steps` lserk steps of the
+// current state in each shape the handle supports -- ms[0] fused stage kernel, ms[1] fused
+// with graph replay, ms[2] face-flux + accumulation kernels, ms[3] those with graph replay
+// (per-step milliseconds, NaN where the shape is unavailable) -- keeps the fastest shape and
+// restores the state, time and residual it started from.
+int teal_gpu_autotune(TealGpu *gpu, int steps, double courant, int time_order, int num_stages,
+                      double *ms);
 
 // Per-kernel timing with HIP events recorded on the library's stream around each launch:
 // teal_gpu_timing(gpu, capacity) preallocates events for `capacity` launches per kernel

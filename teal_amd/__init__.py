@@ -87,6 +87,15 @@ def lib():
         _sig(l, "teal_gpu_timing", C.c_int, VP, C.c_int)
         _sig(l, "teal_gpu_timing_ms", D, VP, C.c_int, IP)
         _sig(l, "teal_gpu_info", C.c_int, VP, LP)
+        _sig(l, "teal_mesh_union", VP, C.POINTER(VP), C.c_int)
+        _sig(l, "teal_gpu_create_batch", VP, C.POINTER(VP), C.c_int, C.c_int)
+        _sig(l, "teal_gpu_batch_size", C.c_int, VP)
+        _sig(l, "teal_gpu_mesh", VP, VP)
+        _sig(l, "teal_gpu_upload_batch", C.c_int, VP, C.POINTER(DP))
+        _sig(l, "teal_gpu_download_batch", C.c_int, VP, C.POINTER(DP))
+        _sig(l, "teal_gpu_run_lserk_batch", C.c_int, VP, C.c_int, DP, DP, D, C.c_int, C.c_int)
+        _sig(l, "teal_gpu_set_graphs", C.c_int, VP, C.c_int)
+        _sig(l, "teal_gpu_autotune", C.c_int, VP, C.c_int, D, C.c_int, C.c_int, DP)
         _lib = l
     return _lib
 
@@ -181,6 +190,24 @@ class Gpu:
             raise RuntimeError(f"implicit_euler failed: {last_error()}")
         return step0, t.value, np.array(res[:])
 
+    def run_lserk(self, steps, time, courant, time_order=2, num_stages=3):
+        t = C.c_double(time)
+        check(self.lib.teal_gpu_run_lserk(self.h, steps, C.byref(t), courant, time_order,
+                                          num_stages), "run_lserk")
+        return t.value
+
+    def set_graphs(self, enable):
+        check(self.lib.teal_gpu_set_graphs(self.h, int(enable)), "set_graphs")
+        return self
+
+    def autotune(self, steps, courant, time_order=2, num_stages=3):
+        """Per-step ms of the launch shapes (fused, fused + graph, split, split + graph);
+        the handle keeps the fastest."""
+        ms = np.zeros(4)
+        check(self.lib.teal_gpu_autotune(self.h, steps, courant, time_order, num_stages, dp(ms)),
+              "autotune")
+        return ms
+
     def krylov_stats(self):
         st = (C.c_long * 2)()
         check(self.lib.teal_gpu_krylov_stats(self.h, st), "krylov_stats")
@@ -207,3 +234,38 @@ class Gpu:
             self.close()
         except Exception:
             pass
+
+
+class Batch(Gpu):
+    """A TealGpu over many independent meshes (teal_gpu_create_batch); configure() sees the
+    union's boundary entities, so a {name: condition} map applies to every instance."""
+
+    def __init__(self, mesh_ptrs, precision=FP64):
+        self.lib = lib()
+        self.meshes = list(mesh_ptrs)
+        arr = (VP * len(self.meshes))(*self.meshes)
+        self.h = self.lib.teal_gpu_create_batch(arr, len(self.meshes), precision)
+        if not self.h:
+            raise RuntimeError(f"teal_gpu_create_batch failed: {last_error()}")
+        self.mesh = self.lib.teal_gpu_mesh(self.h)
+        self.arrays = _abi.mesh_arrays(self.mesh)
+        self.inst_arrays = [_abi.mesh_arrays(m) for m in self.meshes]
+        self.precision = precision
+
+    def upload_batch(self, Ps):
+        Ps = [np.ascontiguousarray(P, dtype=np.float64) for P in Ps]
+        arr = (DP * len(Ps))(*[dp(P) for P in Ps])
+        check(self.lib.teal_gpu_upload_batch(self.h, arr), "upload_batch")
+
+    def download_batch(self):
+        Ps = [np.zeros((a["num_cells"], 5)) for a in self.inst_arrays]
+        arr = (DP * len(Ps))(*[dp(P) for P in Ps])
+        check(self.lib.teal_gpu_download_batch(self.h, arr), "download_batch")
+        return Ps
+
+    def run_lserk_batch(self, steps, times, courant, time_order=2, num_stages=3):
+        t = np.ascontiguousarray(times, dtype=np.float64).copy()
+        res = np.zeros((len(self.meshes), 5))
+        check(self.lib.teal_gpu_run_lserk_batch(self.h, steps, dp(t), dp(res), courant,
+                                                time_order, num_stages), "run_lserk_batch")
+        return t, res

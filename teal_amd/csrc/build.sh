@@ -24,7 +24,7 @@ build_hip device.hip -ffp-contract=off & pids+=($!)
 build_hip mixed.hip -ffp-contract=fast & pids+=($!)
 build_hip krylov.hip -ffp-contract=off & pids+=($!)
 build_hip plugins.hip -ffp-contract=off & pids+=($!)
-for src in core.c sync.c mesh_create.c mesh_generate.c mesh_util.c equations.c simulation.c; do
+for src in core.c sync.c mesh_create.c mesh_generate.c mesh_util.c mesh_union.c equations.c simulation.c; do
     if [ -f "$HERE/$src" ]; then
         gcc "${CFLAGS[@]}" -c "$HERE/$src" -o "$OBJ/${src%.*}.o" & pids+=($!)
     fi

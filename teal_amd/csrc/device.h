@@ -94,10 +94,26 @@ struct EventPool {
 struct Krylov;
 
 struct TealGpu {
-    const Mesh *mesh = nullptr;
+    const Mesh *mesh = nullptr;  // the mesh, or the union of a batch's meshes
     int precision = 0;
-    int ni = 0, nc = 0, nf = 0, nfi = 0, nfb = 0;
+    int ni = 0, nc = 0, nf = 0, nfi = 0, nfb = 0;  // device rows (inner rows include padding)
+    int mesh_ni = 0;                               // inner cells of the mesh
     double sum_volume = 0;
+    // independent instances (a batch): instance i owns mesh inner cells
+    // [inst_cell_off[i], inst_cell_off[i+1]) and device blocks [inst_blk_off[i], ...[i+1]);
+    // its device rows are padded to whole blocks (padding rows have no faces, volume 1 and a
+    // resting state, so they add nothing to the dt minimum or the residual)
+    int ninst = 1;
+    Mesh *owned = nullptr;  // the union mesh of a batch, destroyed with the handle
+    std::vector<int> inst_cell_off, inst_blk_off;
+    std::vector<double> inst_volume;
+    // host rows of instance i in the union: inner, boundary-ghost and periodic-ghost bases and
+    // counts, inst_rows[6 * i + 0..5]
+    std::vector<int> inst_rows;
+    int *d_blk_inst = nullptr;
+    int *d_inst_blk_off = nullptr;
+    int *d_all_blk = nullptr;  // {0, nblocks}: every block as one range
+    double *d_inst_volume = nullptr;
     std::vector<int> periodic_src;
     std::vector<int> cell_perm, cell_inv, face_perm, face_inv;  // device <-> reference numbering
     int *d_cell_perm = nullptr;
@@ -122,6 +138,16 @@ struct TealGpu {
     fv::Scalars *h_scalars = nullptr;
     int timing = 0;
     EventPool ev_flux, ev_grad, ev_face;
+    // graph replay of explicit steps: a captured pair of steps (an even number of stage buffer
+    // swaps, so every replay starts from the buffers it was captured with), valid for the key
+    int graphs = 0;
+    hipGraphExec_t graph = nullptr;
+    struct {
+        double courant;
+        const double *alpha;
+        int stages;
+        const void *P;
+    } graph_key{};
     Krylov *krylov = nullptr;
     int serial_reductions = 0;  // TEAL_REDUCE_SERIAL: GMRES/Newton dot products in reference order
 

@@ -43,8 +43,13 @@ extern "C" const char *teal_gpu_last_error(void) { return g_error; }
     } while (0)
 
 // ------------------------------------------------------------------ small kernels
-__global__ void k_init_scalars(Scalars *sc, double time)
+// one workgroup per instance; times[i] (or `time` for all, if times is null)
+__global__ void k_init_scalars(Scalars *scalars, const double *times, double time)
 {
+    Scalars *sc = &scalars[blockIdx.x];
+    if (times) {
+        time = times[blockIdx.x];
+    }
     sc->dtmin_bits = 0x7ff0000000000000ULL;
     sc->step0 = 0;
     sc->step = 0;
@@ -55,15 +60,20 @@ __global__ void k_init_scalars(Scalars *sc, double time)
 
 // dt minimum over the per-block minima (src/equations/timestep.c:22-29: a min, so the order
 // does not matter); begin != 0 also sets step0 = courant * dt and step = min(step0, max_step)
-// for the step about to run (src/simulation/advance.c:92-93)
+// for the step about to run (src/simulation/advance.c:92-93).  Workgroup i reduces the blocks
+// [blk_off[i], blk_off[i + 1]) of instance i into scalars[i].
 constexpr int DT_THREADS = 1024;
 
-__global__ __launch_bounds__(DT_THREADS) void k_dt_min(Scalars *sc, const double *dtpart, int n,
-                                                      int begin, double courant, double max_step)
+__global__ __launch_bounds__(DT_THREADS) void k_dt_min(Scalars *scalars, const double *dtpart,
+                                                      const int *blk_off, int begin, double courant,
+                                                      double max_step)
 {
     __shared__ double red[DT_THREADS / 64];
+    Scalars *sc = &scalars[blockIdx.x];
+    const int b0 = blk_off[blockIdx.x];
+    const int b1 = blk_off[blockIdx.x + 1];
     double dt = __longlong_as_double(0x7ff0000000000000LL);
-    for (int b = threadIdx.x; b < n; b += DT_THREADS) {
+    for (int b = b0 + threadIdx.x; b < b1; b += DT_THREADS) {
         dt = fmin(dt, dtpart[b]);
     }
     for (int off = 32; off > 0; off >>= 1) dt = fmin(dt, __shfl_xor(dt, off));
@@ -82,20 +92,25 @@ __global__ __launch_bounds__(DT_THREADS) void k_dt_min(Scalars *sc, const double
 }
 
 // residual = sqrt(sum V R^2 / sum V) from per-block partials [nblocks][5] in a fixed order
-// (src/equations/residual.c:8-33), then time += step (advance.c:111).  Thread t of the
-// STEP_END_THREADS = 5 x 128 sums variable t % 5 of blocks t / 5, t / 5 + 128, ... (coalesced
-// reads), then a fixed tree over the 128 partial sums.
+// (src/equations/residual.c:8-33), then time += step (advance.c:111).  Workgroup i handles
+// the blocks [blk_off[i], blk_off[i + 1]) of instance i, of total volume volume[i].  Thread t
+// of the STEP_END_THREADS = 5 x 128 sums variable t % 5 of blocks t / 5, t / 5 + 128, ...
+// (coalesced reads), then a fixed tree over the 128 partial sums.
 constexpr int STEP_END_THREADS = 640;
 
-__global__ __launch_bounds__(STEP_END_THREADS) void k_step_end(Scalars *sc, const double *partials,
-                                                               int nblocks, double sum_volume,
-                                                               int advance_time)
+__global__ __launch_bounds__(STEP_END_THREADS) void k_step_end(Scalars *scalars, const double *partials,
+                                                               const int *blk_off,
+                                                               const double *volume, int advance_time)
 {
     __shared__ double red[STEP_END_THREADS];
+    Scalars *sc = &scalars[blockIdx.x];
+    const double sum_volume = volume[blockIdx.x];
+    const long e0 = (long)blk_off[blockIdx.x] * 5;
+    const long e1 = (long)blk_off[blockIdx.x + 1] * 5;
     const int t = threadIdx.x;
     const int lanes = STEP_END_THREADS / 5;
     double s = 0;
-    for (long e = t; e < (long)nblocks * 5; e += STEP_END_THREADS) {
+    for (long e = e0 + t; e < e1; e += STEP_END_THREADS) {
         s += partials[e];
     }
     red[t] = s;
@@ -150,7 +165,8 @@ __global__ void k_u0_from_host(const double *__restrict__ host, const int *__res
 {
     int d = (blockIdx.x * BLOCK) + threadIdx.x;
     if (d >= ni) return;
-    const double *p = &host[(long)perm[d] * 5];
+    const double rest[5] = {1, 0, 0, 0, 1};  // padding rows of a batch
+    const double *p = (perm[d] < 0) ? rest : &host[(long)perm[d] * 5];
     double u[5];
     prim2cons<double>(u, p, gamma);
     for (int v = 0; v < 5; v++) U0[((long)v * ni) + d] = u[v];
@@ -161,7 +177,7 @@ __global__ void k_prim_from_u(const double *__restrict__ U, const int *__restric
                               double gamma, double *__restrict__ host)
 {
     int d = (blockIdx.x * BLOCK) + threadIdx.x;
-    if (d >= ni) return;
+    if (d >= ni || perm[d] < 0) return;
     double u[5], p[5];
     for (int v = 0; v < 5; v++) u[v] = U[((long)v * ni) + d];
     cons2prim<double>(p, u, gamma);
@@ -235,37 +251,58 @@ void teal_device_numbering(TealGpu *g, const Mesh *m)
     const int ni = m->cells.num_inner;
     const int nc = m->cells.num;
     const int nf = m->faces.num;
-    double lo[3] = {INFINITY, INFINITY, INFINITY};
-    double hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int c = 0; c < ni; c++) {
-        const double *x = &m->cells.center[c].x;
-        for (int d = 0; d < 3; d++) {
-            lo[d] = fmin(lo[d], x[d]);
-            hi[d] = fmax(hi[d], x[d]);
+    // device inner rows: each instance's cells in Morton order of their centres (within the
+    // instance's own bounding box); a batch pads every instance to whole blocks
+    g->inst_blk_off.assign(g->ninst + 1, 0);
+    std::vector<std::pair<uint64_t, int>> key;
+    std::vector<int> rows;  // device row of each mesh inner cell
+    rows.assign(ni, -1);
+    int base = 0;
+    for (int i = 0; i < g->ninst; i++) {
+        int c0 = g->inst_cell_off[i];
+        int c1 = g->inst_cell_off[i + 1];
+        double lo[3] = {INFINITY, INFINITY, INFINITY};
+        double hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int c = c0; c < c1; c++) {
+            const double *x = &m->cells.center[c].x;
+            for (int d = 0; d < 3; d++) {
+                lo[d] = fmin(lo[d], x[d]);
+                hi[d] = fmax(hi[d], x[d]);
+            }
         }
-    }
-    std::vector<std::pair<uint64_t, int>> key(ni);
-    for (int c = 0; c < ni; c++) {
-        const double *x = &m->cells.center[c].x;
-        uint64_t k = 0;
-        for (int d = 0; d < 3; d++) {
-            double span = hi[d] - lo[d];
-            double t = (span > 0) ? (x[d] - lo[d]) / span : 0.0;
-            uint64_t q = (uint64_t)llround(t * 2097151.0);
-            k |= spread_bits(q) << d;
+        key.assign(c1 - c0, {0, 0});
+        for (int c = c0; c < c1; c++) {
+            const double *x = &m->cells.center[c].x;
+            uint64_t k = 0;
+            for (int d = 0; d < 3; d++) {
+                double span = hi[d] - lo[d];
+                double t = (span > 0) ? (x[d] - lo[d]) / span : 0.0;
+                uint64_t q = (uint64_t)llround(t * 2097151.0);
+                k |= spread_bits(q) << d;
+            }
+            key[c - c0] = {k, c};
         }
-        key[c] = {k, c};
+        std::sort(key.begin(), key.end());
+        for (int k = 0; k < c1 - c0; k++) {
+            rows[key[k].second] = base + k;
+        }
+        int padded = (g->ninst == 1) ? (c1 - c0) : ((c1 - c0 + BLOCK - 1) / BLOCK) * BLOCK;
+        base += padded;
+        g->inst_blk_off[i + 1] = (base + BLOCK - 1) / BLOCK;
     }
-    std::sort(key.begin(), key.end());
-    g->cell_perm.assign(nc, 0);
+    const int dni = base;  // device inner rows
+    g->mesh_ni = ni;
+    g->ni = dni;
+    g->nc = dni + (nc - ni);
+    g->cell_perm.assign(g->nc, -1);
     g->cell_inv.assign(nc, 0);
-    for (int d = 0; d < ni; d++) {
-        g->cell_perm[d] = key[d].second;
-        g->cell_inv[key[d].second] = d;
+    for (int c = 0; c < ni; c++) {
+        g->cell_perm[rows[c]] = c;
+        g->cell_inv[c] = rows[c];
     }
-    for (int c = ni; c < nc; c++) {
-        g->cell_perm[c] = c;
-        g->cell_inv[c] = c;
+    for (int c = ni; c < nc; c++) {  // ghosts keep their order after the inner rows
+        g->cell_perm[dni + (c - ni)] = c;
+        g->cell_inv[c] = dni + (c - ni);
     }
     std::vector<std::pair<long, int>> fkey(nf);
     for (int f = 0; f < nf; f++) {
@@ -283,8 +320,8 @@ void teal_device_numbering(TealGpu *g, const Mesh *m)
 template <typename S>
 int DevData<S>::build(const Mesh *m, TealGpu *gpu)
 {
-    const int ni = m->cells.num_inner;
-    const int nc = m->cells.num;
+    const int ni = gpu->ni;  // device rows (padding included)
+    const int nc = gpu->nc;
     const int nf = m->faces.num;
     const int nfi = m->faces.num_inner;
     const int nfb = m->faces.off_boundary;
@@ -331,9 +368,12 @@ int DevData<S>::build(const Mesh *m, TealGpu *gpu)
             }
         }
     }
-    std::vector<double> vol(ni), proj((size_t)3 * ni);
+    std::vector<double> vol(ni, 1.0), proj((size_t)3 * ni, 0.0);  // padding rows: 1 and 0
     for (int d = 0; d < ni; d++) {
         int c = cperm[d];
+        if (c < 0) {
+            continue;
+        }
         vol[d] = m->cells.volume[c];
         proj[d] = m->cells.projection[c].x;
         proj[ni + d] = m->cells.projection[c].y;
@@ -343,8 +383,11 @@ int DevData<S>::build(const Mesh *m, TealGpu *gpu)
         corr((size_t)4 * nf);
     std::vector<int2> flr(nf);
     std::vector<int> crow(nf);
-    std::vector<double> fcen((size_t)3 * nf), ccen((size_t)3 * nc);
+    std::vector<double> fcen((size_t)3 * nf), ccen((size_t)3 * nc, 0.0);
     for (int d = 0; d < nc; d++) {
+        if (cperm[d] < 0) {
+            continue;
+        }
         const double *x = &m->cells.center[cperm[d]].x;
         for (int k = 0; k < 3; k++) ccen[((size_t)d * 3) + k] = x[k];
     }
@@ -352,7 +395,7 @@ int DevData<S>::build(const Mesh *m, TealGpu *gpu)
         int f = fperm[d];
         int r = m->faces.cell_idx[f].right;
         flr[d].x = cinv[m->faces.cell_idx[f].left];
-        flr[d].y = (f >= nfi && f < nfb) ? ~r : resolve(r);
+        flr[d].y = (f >= nfi && f < nfb) ? ~cinv[r] : resolve(r);
         crow[d] = cinv[r];  // ghosts keep their row; a periodic face's own ghost, not its source
         const double *fc = &m->faces.center[f].x;
         for (int k = 0; k < 3; k++) fcen[((size_t)k * nf) + d] = fc[k];
@@ -383,7 +426,7 @@ int DevData<S>::build(const Mesh *m, TealGpu *gpu)
     for (int i = 0; i < nb; i++) {
         bf[i] = finv[nfi + i];
         bl[i] = cinv[m->faces.cell_idx[nfi + i].left];
-        br[i] = m->faces.cell_idx[nfi + i].right;
+        br[i] = cinv[m->faces.cell_idx[nfi + i].right];
     }
     if (upload(&d_bleft, bl) || upload(&d_bright, br) || upload(&d_bface, bf)) {
         return -1;
@@ -569,32 +612,24 @@ template struct DevData<float>;
 
 
 // ------------------------------------------------------------------ TealGpu
-extern "C" TealGpu *teal_gpu_create(const Mesh *mesh, int precision)
+// a handle over `mesh` whose inner cells split into `ninst` independent instances at
+// inst_cell_off (one instance: the whole mesh)
+static TealGpu *create_handle(const Mesh *mesh, int precision, int ninst, const int *inst_cell_off,
+                              const double *inst_volume, Mesh *owned)
 {
-    if (!mesh || !mesh->faces.area) {
-        teal_gpu_set_error("teal_gpu_create: mesh is not generated");
-        return nullptr;
-    }
-    if (precision != TEAL_FP64 && precision != TEAL_MIXED) {
-        teal_gpu_set_error("teal_gpu_create: invalid precision (%d)", precision);
-        return nullptr;
-    }
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
-        teal_gpu_set_error("teal_gpu_create: no HIP device is available");
-        return nullptr;
-    }
     TealGpu *g = new TealGpu();
     g->mesh = mesh;
+    g->owned = owned;
     g->precision = precision;
-    g->ni = mesh->cells.num_inner;
-    g->nc = mesh->cells.num;
+    g->ninst = ninst;
+    g->inst_cell_off.assign(inst_cell_off, inst_cell_off + ninst + 1);
+    g->inst_volume.assign(inst_volume, inst_volume + ninst);
     g->nf = mesh->faces.num;
     g->nfi = mesh->faces.num_inner;
     g->nfb = mesh->faces.off_boundary;
     g->sum_volume = mesh->cells.sum_volume;
     g->periodic_src = periodic_sources(mesh);
-    teal_device_numbering(g, mesh);
+    teal_device_numbering(g, mesh);  // sets ni, nc (device rows), mesh_ni, instance blocks
     int rc = (precision == TEAL_FP64) ? g->d64.build(mesh, g) : g->d32.build(mesh, g);
     if (rc == 0) {  // the fused stage kernel needs a block's face list in LDS
         int dev = 0, lds_max = 0;
@@ -608,28 +643,38 @@ extern "C" TealGpu *teal_gpu_create(const Mesh *mesh, int precision)
         g->d64.fused = g->d64.block_lds() > 0 && g->d64.block_lds() <= (size_t)g->lds_max;
         g->d32.fused = g->d32.block_lds() > 0 && g->d32.block_lds() <= (size_t)g->lds_max;
     }
-    if (rc == 0 && upload(&g->d_cell_perm, g->cell_perm)) {
+    g->nblocks = grid_of(g->ni);
+    std::vector<int> blk_inst(g->nblocks, 0);
+    for (int i = 0; i < ninst; i++) {
+        for (int b = g->inst_blk_off[i]; b < g->inst_blk_off[i + 1]; b++) blk_inst[b] = i;
+    }
+    std::vector<int> all_blk = {0, g->nblocks};
+    if (rc == 0 && (upload(&g->d_cell_perm, g->cell_perm) || upload(&g->d_blk_inst, blk_inst) ||
+                    upload(&g->d_inst_blk_off, g->inst_blk_off) || upload(&g->d_all_blk, all_blk) ||
+                    upload(&g->d_inst_volume, g->inst_volume))) {
         rc = -1;
     }
     if (rc == 0 && hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess) {
         teal_gpu_set_error("teal_gpu_create: hipStreamCreate failed");
         rc = -1;
     }
-    size_t stage = (size_t)15 * (g->nc > g->ni ? g->nc : g->ni);
-    g->nblocks = grid_of(g->ni);
+    // host-layout staging: up to 15 values per mesh row, or 5 per device row
+    size_t stage = std::max((size_t)15 * mesh->cells.num, (size_t)5 * g->nc);
     if (rc == 0 &&
         (hipMalloc((void **)&g->d_stage, sizeof(double) * stage) != hipSuccess ||
          hipMalloc((void **)&g->d_U0, sizeof(double) * 5 * (size_t)g->ni) != hipSuccess ||
          hipMalloc((void **)&g->d_R, sizeof(double) * 5 * (size_t)g->ni) != hipSuccess ||
          hipMalloc((void **)&g->d_partials, sizeof(double) * 5 * (size_t)g->nblocks) != hipSuccess ||
          hipMalloc((void **)&g->d_dtpart, sizeof(double) * (size_t)g->nblocks) != hipSuccess ||
-         hipMalloc((void **)&g->d_scalars, sizeof(Scalars)) != hipSuccess ||
-         hipHostMalloc((void **)&g->h_scalars, sizeof(Scalars), hipHostMallocDefault) != hipSuccess)) {
+         hipMalloc((void **)&g->d_scalars, sizeof(Scalars) * (size_t)ninst) != hipSuccess ||
+         hipHostMalloc((void **)&g->h_scalars, sizeof(Scalars) * (size_t)ninst, hipHostMallocDefault) !=
+             hipSuccess)) {
         teal_gpu_set_error("teal_gpu_create: device allocation failed");
         rc = -1;
     }
     if (rc == 0) {
-        hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(1), 0, g->stream, g->d_scalars, 0.0);
+        hipLaunchKernelGGL(k_init_scalars, dim3(ninst), dim3(1), 0, g->stream, g->d_scalars,
+                           (const double *)nullptr, 0.0);
         if (hipStreamSynchronize(g->stream) != hipSuccess) {
             teal_gpu_set_error("teal_gpu_create: initialization kernel failed");
             rc = -1;
@@ -638,6 +683,83 @@ extern "C" TealGpu *teal_gpu_create(const Mesh *mesh, int precision)
     if (rc != 0) {
         teal_gpu_destroy(g);
         return nullptr;
+    }
+    return g;
+}
+
+static int device_ready(const char *who)
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        teal_gpu_set_error("%s: no HIP device is available", who);
+        return -1;
+    }
+    return 0;
+}
+
+extern "C" TealGpu *teal_gpu_create(const Mesh *mesh, int precision)
+{
+    if (!mesh || !mesh->faces.area) {
+        teal_gpu_set_error("teal_gpu_create: mesh is not generated");
+        return nullptr;
+    }
+    if (precision != TEAL_FP64 && precision != TEAL_MIXED) {
+        teal_gpu_set_error("teal_gpu_create: invalid precision (%d)", precision);
+        return nullptr;
+    }
+    if (device_ready("teal_gpu_create")) {
+        return nullptr;
+    }
+    int off[2] = {0, mesh->cells.num_inner};
+    double vol = mesh->cells.sum_volume;
+    return create_handle(mesh, precision, 1, off, &vol, nullptr);
+}
+
+extern "C" TealGpu *teal_gpu_create_batch(const Mesh *const *meshes, int num, int precision)
+{
+    if (!meshes || num < 1) {
+        teal_gpu_set_error("teal_gpu_create_batch: no meshes (%d)", num);
+        return nullptr;
+    }
+    if (precision != TEAL_FP64 && precision != TEAL_MIXED) {
+        teal_gpu_set_error("teal_gpu_create_batch: invalid precision (%d)", precision);
+        return nullptr;
+    }
+    for (int i = 0; i < num; i++) {
+        if (!meshes[i] || !meshes[i]->faces.area) {
+            teal_gpu_set_error("teal_gpu_create_batch: mesh %d is not generated", i);
+            return nullptr;
+        }
+    }
+    if (device_ready("teal_gpu_create_batch")) {
+        return nullptr;
+    }
+    std::vector<int> off(num + 1, 0);
+    std::vector<double> vol(num);
+    for (int i = 0; i < num; i++) {
+        off[i + 1] = off[i] + meshes[i]->cells.num_inner;
+        vol[i] = meshes[i]->cells.sum_volume;
+    }
+    Mesh *u = teal_mesh_union(meshes, num);
+    TealGpu *g = create_handle(u, precision, num, off.data(), vol.data(), u);
+    if (!g) {
+        return nullptr;
+    }
+    // host rows of each instance in the union (the group order of teal_mesh_union)
+    g->inst_rows.assign((size_t)6 * num, 0);
+    int inner = 0, boundary = u->cells.num_inner, periodic = u->cells.off_boundary;
+    for (int i = 0; i < num; i++) {
+        const Mesh *m = meshes[i];
+        int *r = &g->inst_rows[(size_t)6 * i];
+        r[0] = inner;
+        r[1] = m->cells.num_inner;
+        r[2] = boundary;
+        r[3] = m->cells.off_boundary - m->cells.num_inner;
+        r[4] = periodic;
+        r[5] = m->cells.num - m->cells.off_boundary;
+        inner += r[1];
+        boundary += r[3];
+        periodic += r[5];
     }
     return g;
 }
@@ -653,7 +775,8 @@ extern "C" void teal_gpu_destroy(TealGpu *g)
     g->d64.release();
     g->d32.release();
     void *ptrs[] = {g->d_stage, g->d_U0, g->d_R, g->d_partials, g->d_dtpart, g->d_scalars,
-                    g->d_cell_perm};
+                    g->d_cell_perm, g->d_blk_inst, g->d_inst_blk_off, g->d_all_blk,
+                    g->d_inst_volume};
     for (void *p : ptrs) {
         if (p) {
             (void)hipFree(p);
@@ -668,6 +791,12 @@ extern "C" void teal_gpu_destroy(TealGpu *g)
     teal_krylov_release(g);
     if (g->stream) {
         (void)hipStreamDestroy(g->stream);
+    }
+    if (g->graph) {
+        (void)hipGraphExecDestroy(g->graph);
+    }
+    if (g->owned) {
+        mesh_destroy(g->owned);
     }
     delete g;
 }
@@ -707,6 +836,10 @@ static int configure_impl(TealGpu *g, DevData<S> &d, const TealGpuConfig *cfg)
         std::vector<S> e2(g->ni);
         for (int d = 0; d < g->ni; d++) {  // src/equations/limiter.c:8-21
             int c = g->cell_perm[d];
+            if (c < 0) {
+                e2[d] = S(0);
+                continue;
+            }
             double v = cfg->coefficient * cbrt(m->cells.volume[c]);
             e2[d] = cfg->eps2 ? (S)cfg->eps2[c] : (S)(v * v * v);
         }
@@ -745,9 +878,13 @@ extern "C" int teal_gpu_configure(TealGpu *g, const TealGpuConfig *cfg)
     g->cfg_bc.assign(cfg->bc ? cfg->bc : nullptr, cfg->bc ? cfg->bc + nent : nullptr);
     g->cfg_ctx.assign(cfg->bc_context ? cfg->bc_context : nullptr,
                       cfg->bc_context ? cfg->bc_context + (5 * nent) : nullptr);
-    g->cfg_eps2.assign(cfg->eps2 ? cfg->eps2 : nullptr, cfg->eps2 ? cfg->eps2 + g->ni : nullptr);
+    g->cfg_eps2.assign(cfg->eps2 ? cfg->eps2 : nullptr, cfg->eps2 ? cfg->eps2 + g->mesh_ni : nullptr);
     g->configured = 1;
     g->u0_valid = 0;
+    if (g->graph) {  // captured launches carry the old configuration
+        (void)hipGraphExecDestroy(g->graph);
+        g->graph = nullptr;
+    }
     if (g->precision == TEAL_FP64) {
         return configure_impl(g, g->d64, cfg);
     }
@@ -809,6 +946,7 @@ int teal_residual_pass(TealGpu *g, DevData<S> &d, int mode, double alpha, int la
     a.U0_next = g->d_U0;
     a.alpha = alpha;
     a.scalars = g->d_scalars;
+    a.blk_inst = (g->ninst > 1) ? g->d_blk_inst : nullptr;
     a.partials = g->d_partials;
     a.dtpart = g->d_dtpart;
     a.keep_u = (g->precision == TEAL_MIXED);
@@ -857,10 +995,17 @@ template int teal_residual_pass<float>(TealGpu *, DevData<float> &, int, double,
 
 // host primitive rows (reference numbering) -> device P records, and U0 = prim2cons(P) in FP64
 // (valid for the next explicit step; the time step still has to be computed)
+// host rows of a device-row count: ni or nc device rows hold mesh_ni or the mesh's number of
+// cells (padding rows of a batch have no host row)
+static inline size_t host_rows(const TealGpu *g, int device_rows)
+{
+    return (size_t)device_rows - (size_t)(g->ni - g->mesh_ni);
+}
+
 template <typename S>
 static int upload_primitive(TealGpu *g, DevData<S> &d, const double *primitive)
 {
-    size_t bytes = sizeof(double) * 5 * (size_t)g->nc;
+    size_t bytes = sizeof(double) * 5 * host_rows(g, g->nc);
     HIPCALL(hipMemcpyAsync(g->d_stage, primitive, bytes, hipMemcpyHostToDevice, g->stream));
     Kern<S>::to_device(g->stream, g->d_stage, d.d_P, g->nc, 5, 5, 5, g->d_cell_perm, 0);
     if (g->configured) {
@@ -882,7 +1027,7 @@ static int download_rows(TealGpu *g, const S *src, int num, int src_stride, int 
     Kern<S>::to_host(g->stream, src, g->d_stage, num, src_stride, width, host_stride, g->d_cell_perm,
                      soa);
     HIPCALL(hipGetLastError());
-    HIPCALL(hipMemcpyAsync(host, g->d_stage, sizeof(double) * host_stride * (size_t)num,
+    HIPCALL(hipMemcpyAsync(host, g->d_stage, sizeof(double) * host_stride * host_rows(g, num),
                            hipMemcpyDeviceToHost, g->stream));
     HIPCALL(hipStreamSynchronize(g->stream));
     return 0;
@@ -905,7 +1050,7 @@ static int check_ready(TealGpu *g)
 // the BC pass, periodic ghosts by the exchange (src/equations/exchange.c:8-51)
 static void fill_periodic_ghosts(TealGpu *g, double *buf, int len)
 {
-    for (int c = g->mesh->cells.off_boundary; c < g->nc; c++) {
+    for (int c = g->mesh->cells.off_boundary; c < g->mesh->cells.num; c++) {
         int s = g->periodic_src[c];
         if (s >= 0) {
             memcpy(&buf[(size_t)c * len], &buf[(size_t)s * len], sizeof(double) * len);
@@ -921,9 +1066,9 @@ static int derivative_impl(TealGpu *g, DevData<S> &d, double *primitive, double 
     // ghost states back to the caller (boundary ghosts computed on the device)
     int nb = g->nfb - g->nfi;
     if (nb > 0) {
-        std::vector<double> tmp((size_t)5 * g->nc);
+        std::vector<double> tmp((size_t)5 * g->mesh->cells.num);
         if (download_rows(g, d.d_P, g->nc, 5, 5, 0, 5, tmp.data())) return -1;
-        for (int c = g->ni; c < g->mesh->cells.off_boundary; c++) {
+        for (int c = g->mesh_ni; c < g->mesh->cells.off_boundary; c++) {
             memcpy(&primitive[(size_t)c * 5], &tmp[(size_t)c * 5], sizeof(double) * 5);
         }
     }
@@ -948,9 +1093,9 @@ static int boundary_impl(TealGpu *g, DevData<S> &d, double *primitive)
     if (nb > 0) {
         Kern<S>::boundary(g->stream, nb, g->nf, d, g->cfg.gamma);
         HIPCALL(hipGetLastError());
-        std::vector<double> tmp((size_t)5 * g->nc);
+        std::vector<double> tmp((size_t)5 * g->mesh->cells.num);
         if (download_rows(g, d.d_P, g->nc, 5, 5, 0, 5, tmp.data())) return -1;
-        for (int c = g->ni; c < g->mesh->cells.off_boundary; c++) {
+        for (int c = g->mesh_ni; c < g->mesh->cells.off_boundary; c++) {
             memcpy(&primitive[(size_t)c * 5], &tmp[(size_t)c * 5], sizeof(double) * 5);
         }
     }
@@ -975,23 +1120,23 @@ static int gradient_impl(TealGpu *g, DevData<S> &d, double *primitive, double *g
             teal_gpu_set_error("invalid limiter kind (0)");
             return -1;
         }
-        size_t bytes = sizeof(double) * 15 * (size_t)g->nc;
+        size_t bytes = sizeof(double) * 15 * (size_t)g->mesh->cells.num;
         HIPCALL(hipMemcpyAsync(g->d_stage, gradient, bytes, hipMemcpyHostToDevice, g->stream));
         Kern<S>::to_device(g->stream, g->d_stage, d.d_GP, g->ni, 15, 15, 20, g->d_cell_perm, 0);
     }
     Kern<S>::grad_limit(g->stream, lim, mode, g->topo(), d.geom(), d.d_P, d.d_GP);
     HIPCALL(hipGetLastError());
     // rows of 20: gradient (15) then psi (5); the limited gradient is G * psi (limiter.c:61-63)
-    std::vector<double> GP((size_t)20 * g->ni);
+    std::vector<double> GP((size_t)20 * g->mesh_ni);
     if (download_rows(g, d.d_GP, g->ni, 20, 20, 0, 20, GP.data())) return -1;
-    for (int c = 0; c < g->ni; c++) {
+    for (int c = 0; c < g->mesh_ni; c++) {
         const double *row = &GP[(size_t)c * 20];
         for (int i = 0; i < 15; i++) {
             gradient[((size_t)c * 15) + i] = (mode == 1) ? row[i] * row[15 + (i / 3)] : row[i];
         }
     }
     if (mode == 0) {
-        for (int c = g->ni; c < g->nc; c++) {
+        for (int c = g->mesh_ni; c < g->mesh->cells.num; c++) {
             memset(&gradient[(size_t)c * 15], 0, sizeof(double) * 15);
         }
         fill_periodic_ghosts(g, gradient, 15);
@@ -1011,7 +1156,7 @@ extern "C" int teal_gpu_gradient(TealGpu *g, double *primitive, double *gradient
 extern "C" int teal_gpu_limiter(TealGpu *g, const double *primitive, double *gradient)
 {
     if (check_ready(g)) return -1;
-    std::vector<double> p(primitive, primitive + (size_t)5 * g->nc);
+    std::vector<double> p(primitive, primitive + (size_t)5 * g->mesh->cells.num);
     return (g->precision == TEAL_FP64) ? gradient_impl(g, g->d64, p.data(), gradient, 1)
                                        : gradient_impl(g, g->d32, p.data(), gradient, 1);
 }
@@ -1023,8 +1168,9 @@ int teal_timestep_pass(TealGpu *g, DevData<S> &d, int write_u0, double *out)
                          write_u0);
     HIPCALL(hipGetLastError());
     if (out) {
+        // one minimum over all rows (a batch handle answers for the union of its meshes)
         hipLaunchKernelGGL(k_dt_min, dim3(1), dim3(DT_THREADS), 0, g->stream, g->d_scalars,
-                           (const double *)g->d_dtpart, g->nblocks, 0, 0.0, 0.0);
+                           (const double *)g->d_dtpart, (const int *)g->d_all_blk, 0, 0.0, 0.0);
         HIPCALL(hipMemcpyAsync(g->h_scalars, g->d_scalars, sizeof(Scalars), hipMemcpyDeviceToHost,
                                g->stream));
         HIPCALL(hipStreamSynchronize(g->stream));
@@ -1036,7 +1182,7 @@ int teal_timestep_pass(TealGpu *g, DevData<S> &d, int write_u0, double *out)
 extern "C" double teal_gpu_timestep(TealGpu *g, const double *primitive)
 {
     if (check_ready(g)) return NAN;
-    std::vector<double> p(primitive, primitive + (size_t)5 * g->nc);
+    std::vector<double> p(primitive, primitive + (size_t)5 * g->mesh->cells.num);
     double dt = NAN;
     int rc = (g->precision == TEAL_FP64)
                  ? (upload_primitive(g, g->d64, p.data()) || teal_timestep_pass(g, g->d64, 0, &dt))
@@ -1045,19 +1191,28 @@ extern "C" double teal_gpu_timestep(TealGpu *g, const double *primitive)
     return rc ? NAN : dt;
 }
 
+// per-instance residual norms of the partials in d_partials: residual[5 * i + v]
+static int read_residuals(TealGpu *g, double *residual)
+{
+    hipLaunchKernelGGL(k_step_end, dim3(g->ninst), dim3(STEP_END_THREADS), 0, g->stream,
+                       g->d_scalars, (const double *)g->d_partials, (const int *)g->d_inst_blk_off,
+                       (const double *)g->d_inst_volume, 0);
+    HIPCALL(hipGetLastError());
+    HIPCALL(hipMemcpyAsync(g->h_scalars, g->d_scalars, sizeof(Scalars) * (size_t)g->ninst,
+                           hipMemcpyDeviceToHost, g->stream));
+    HIPCALL(hipStreamSynchronize(g->stream));
+    for (int i = 0; i < g->ninst; i++) {
+        memcpy(&residual[(size_t)i * 5], g->h_scalars[i].residual, sizeof(double) * 5);
+    }
+    return 0;
+}
+
 int teal_residual_norm(TealGpu *g, const double *d_derivative, double *residual)
 {
     const double *vol64 = (g->precision == TEAL_FP64) ? g->d64.d_vol64 : g->d32.d_vol64;
     hipLaunchKernelGGL(k_residual_partials, dim3(g->nblocks), dim3(BLOCK), 0, g->stream, d_derivative,
                        vol64, g->ni, g->d_partials);
-    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(STEP_END_THREADS), 0, g->stream, g->d_scalars,
-                       (const double *)g->d_partials, g->nblocks, g->sum_volume, 0);
-    HIPCALL(hipGetLastError());
-    HIPCALL(hipMemcpyAsync(g->h_scalars, g->d_scalars, sizeof(Scalars), hipMemcpyDeviceToHost,
-                           g->stream));
-    HIPCALL(hipStreamSynchronize(g->stream));
-    memcpy(residual, g->h_scalars->residual, sizeof(double) * 5);
-    return 0;
+    return read_residuals(g, residual);
 }
 
 extern "C" int teal_gpu_residual(TealGpu *g, const double *derivative, double *residual)
@@ -1066,20 +1221,13 @@ extern "C" int teal_gpu_residual(TealGpu *g, const double *derivative, double *r
         teal_gpu_set_error("null TealGpu");
         return -1;
     }
-    HIPCALL(hipMemcpyAsync(g->d_stage, derivative, sizeof(double) * 5 * (size_t)g->ni,
+    HIPCALL(hipMemcpyAsync(g->d_stage, derivative, sizeof(double) * 5 * (size_t)g->mesh_ni,
                            hipMemcpyHostToDevice, g->stream));
     Kern<double>::to_device(g->stream, g->d_stage, g->d_R, g->ni, 5, 5, 5, g->d_cell_perm, 1);
     const double *vol64 = (g->precision == TEAL_FP64) ? g->d64.d_vol64 : g->d32.d_vol64;
     hipLaunchKernelGGL(k_residual_partials, dim3(g->nblocks), dim3(BLOCK), 0, g->stream, g->d_R,
                        vol64, g->ni, g->d_partials);
-    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(STEP_END_THREADS), 0, g->stream, g->d_scalars,
-                       (const double *)g->d_partials, g->nblocks, g->sum_volume, 0);
-    HIPCALL(hipGetLastError());
-    HIPCALL(hipMemcpyAsync(g->h_scalars, g->d_scalars, sizeof(Scalars), hipMemcpyDeviceToHost,
-                           g->stream));
-    HIPCALL(hipStreamSynchronize(g->stream));
-    memcpy(residual, g->h_scalars->residual, sizeof(double) * 5);
-    return 0;
+    return read_residuals(g, residual);
 }
 
 // ------------------------------------------------------------------ resident state
@@ -1106,12 +1254,13 @@ extern "C" int teal_gpu_download(TealGpu *g, double *primitive)
                            (const double *)g->d_U0, (const int *)g->d_cell_perm, g->ni,
                            g->cfg.gamma, g->d_stage);
         HIPCALL(hipGetLastError());
-        std::vector<double> tmp((size_t)5 * g->ni);
-        HIPCALL(hipMemcpyAsync(tmp.data(), g->d_stage, sizeof(double) * 5 * (size_t)g->ni,
+        std::vector<double> tmp((size_t)5 * g->mesh_ni);
+        HIPCALL(hipMemcpyAsync(tmp.data(), g->d_stage, sizeof(double) * 5 * (size_t)g->mesh_ni,
                                hipMemcpyDeviceToHost, g->stream));
         HIPCALL(hipStreamSynchronize(g->stream));
         for (int d = 0; d < g->ni; d++) {
             int c = g->cell_perm[d];
+            if (c < 0) continue;  // padding row of a batch
             memcpy(&primitive[(size_t)c * 5], &tmp[(size_t)c * 5], sizeof(double) * 5);
         }
     }
@@ -1147,6 +1296,25 @@ static const double lserk_alpha[3][6][7] = {
     },
 };
 
+// the launches of one explicit step once U0 and the per-block dt minima are current: per
+// instance dt, the stages, per-instance residual and time (what a graph of a step holds)
+template <typename S>
+static int explicit_stages(TealGpu *g, DevData<S> &d, double max_step, double courant,
+                           const double *alpha, int num_stages)
+{
+    hipLaunchKernelGGL(k_dt_min, dim3(g->ninst), dim3(DT_THREADS), 0, g->stream, g->d_scalars,
+                       (const double *)g->d_dtpart, (const int *)g->d_inst_blk_off, 1, courant,
+                       max_step);
+    for (int s = 0; s < num_stages; s++) {
+        if (teal_residual_pass(g, d, 1, alpha[s + 1], s == num_stages - 1)) return -1;
+    }
+    hipLaunchKernelGGL(k_step_end, dim3(g->ninst), dim3(STEP_END_THREADS), 0, g->stream,
+                       g->d_scalars, (const double *)g->d_partials, (const int *)g->d_inst_blk_off,
+                       (const double *)g->d_inst_volume, 1);
+    HIPCALL(hipGetLastError());
+    return 0;
+}
+
 // one explicit step on the device; nothing is read back
 template <typename S>
 static int explicit_step(TealGpu *g, DevData<S> &d, double max_step, double courant,
@@ -1157,21 +1325,13 @@ static int explicit_step(TealGpu *g, DevData<S> &d, double max_step, double cour
         g->u0_valid = 1;
         g->dt_valid = 1;
     }
-    hipLaunchKernelGGL(k_dt_min, dim3(1), dim3(DT_THREADS), 0, g->stream, g->d_scalars,
-                       (const double *)g->d_dtpart, g->nblocks, 1, courant, max_step);
-    for (int s = 0; s < num_stages; s++) {
-        if (teal_residual_pass(g, d, 1, alpha[s + 1], s == num_stages - 1)) return -1;
-    }
-    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(STEP_END_THREADS), 0, g->stream, g->d_scalars,
-                       (const double *)g->d_partials, g->nblocks, g->sum_volume, 1);
-    HIPCALL(hipGetLastError());
-    return 0;
+    return explicit_stages(g, d, max_step, courant, alpha, num_stages);
 }
 
 static int read_scalars(TealGpu *g)
 {
-    HIPCALL(hipMemcpyAsync(g->h_scalars, g->d_scalars, sizeof(Scalars), hipMemcpyDeviceToHost,
-                           g->stream));
+    HIPCALL(hipMemcpyAsync(g->h_scalars, g->d_scalars, sizeof(Scalars) * (size_t)g->ninst,
+                           hipMemcpyDeviceToHost, g->stream));
     HIPCALL(hipStreamSynchronize(g->stream));
     if (g->h_scalars->error == 1) {
         teal_gpu_set_error("vacuum state detected in riemann solver");
@@ -1184,10 +1344,20 @@ static int read_scalars(TealGpu *g)
     return 0;
 }
 
+static int check_single(TealGpu *g, const char *who)
+{
+    if (check_ready(g)) return -1;
+    if (g->ninst > 1) {
+        teal_gpu_set_error("%s: a batch handle advances with teal_gpu_run_lserk_batch", who);
+        return -1;
+    }
+    return 0;
+}
+
 extern "C" double teal_gpu_lserk(TealGpu *g, double *time, double *residual, double max_step,
                                  double courant, int time_order, int num_stages)
 {
-    if (check_ready(g)) return NAN;
+    if (check_single(g, "lserk")) return NAN;
     if (time_order < 1 || time_order > 3 || num_stages < 1 || num_stages > 6) {
         teal_gpu_set_error("lserk: invalid time order (%d) / stages (%d)", time_order, num_stages);
         return NAN;
@@ -1207,7 +1377,7 @@ extern "C" double teal_gpu_lserk(TealGpu *g, double *time, double *residual, dou
 extern "C" double teal_gpu_euler(TealGpu *g, double *time, double *residual, double max_step,
                                  double courant)
 {
-    if (check_ready(g)) return NAN;
+    if (check_single(g, "euler")) return NAN;
     static const double alpha[2] = {0.0, 1.0};
     int rc = (g->precision == TEAL_FP64) ? explicit_step(g, g->d64, max_step, courant, alpha, 1)
                                          : explicit_step(g, g->d32, max_step, courant, alpha, 1);
@@ -1217,26 +1387,323 @@ extern "C" double teal_gpu_euler(TealGpu *g, double *time, double *residual, dou
     return g->h_scalars->step0;
 }
 
+static void drop_graph(TealGpu *g)
+{
+    if (g->graph) {
+        (void)hipGraphExecDestroy(g->graph);
+        g->graph = nullptr;
+    }
+}
+
+// the graph of two explicit steps with these coefficients, captured on first use (capture
+// records the launches without running them; the stage buffers end where they started)
+template <typename S>
+static int step_graph(TealGpu *g, DevData<S> &d, double courant, const double *alpha,
+                      int num_stages)
+{
+    if (g->graph && g->graph_key.courant == courant && g->graph_key.alpha == alpha &&
+        g->graph_key.stages == num_stages && g->graph_key.P == (const void *)d.d_P) {
+        return 0;
+    }
+    drop_graph(g);
+    const void *P = d.d_P;
+    HIPCALL(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int k = 0; k < 2 && rc == 0; k++) {
+        rc = explicit_stages(g, d, INFINITY, courant, alpha, num_stages);
+    }
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(g->stream, &graph);
+    if (rc == 0 && e == hipSuccess && (const void *)d.d_P != P) {
+        teal_gpu_set_error("step graph: stage buffers did not return to their start");
+        rc = -1;
+    }
+    if (rc == 0 && e == hipSuccess) {
+        e = hipGraphInstantiate(&g->graph, graph, nullptr, nullptr, 0);
+    }
+    if (graph) {
+        (void)hipGraphDestroy(graph);
+    }
+    if (rc || e != hipSuccess) {
+        if (rc == 0) {
+            teal_gpu_set_error("step graph capture failed: %s", hipGetErrorString(e));
+        }
+        g->graph = nullptr;
+        return -1;
+    }
+    g->graph_key.courant = courant;
+    g->graph_key.alpha = alpha;
+    g->graph_key.stages = num_stages;
+    g->graph_key.P = P;
+    return 0;
+}
+
+// `steps` explicit steps without read-back: the first eagerly when U0 or the dt minima have
+// to be computed, then graph replays of step pairs (when enabled), the rest eagerly
+template <typename S>
+static int run_steps(TealGpu *g, DevData<S> &d, int steps, double courant, const double *alpha,
+                     int num_stages)
+{
+    int done = 0;
+    if (steps > 0 && (!g->u0_valid || !g->dt_valid)) {
+        if (explicit_step(g, d, INFINITY, courant, alpha, num_stages)) return -1;
+        done = 1;
+    }
+    int pairs = (g->graphs && !g->timing) ? (steps - done) / 2 : 0;
+    if (pairs > 0) {
+        if (step_graph(g, d, courant, alpha, num_stages)) return -1;
+        for (int k = 0; k < pairs; k++) {
+            HIPCALL(hipGraphLaunch(g->graph, g->stream));
+        }
+        done += 2 * pairs;
+    }
+    for (; done < steps; done++) {
+        if (explicit_step(g, d, INFINITY, courant, alpha, num_stages)) return -1;
+    }
+    return 0;
+}
+
+static const double *lserk_coefficients(int time_order, int num_stages)
+{
+    if (time_order < 1 || time_order > 3 || num_stages < 1 || num_stages > 6) {
+        teal_gpu_set_error("lserk: invalid time order (%d) / stages (%d)", time_order, num_stages);
+        return nullptr;
+    }
+    return lserk_alpha[time_order - 1][num_stages - 1];
+}
+
 extern "C" int teal_gpu_run_lserk(TealGpu *g, int steps, double *time, double courant,
                                   int time_order, int num_stages)
 {
-    if (check_ready(g)) return -1;
-    if (time_order < 1 || time_order > 3 || num_stages < 1 || num_stages > 6) {
-        teal_gpu_set_error("lserk: invalid time order (%d) / stages (%d)", time_order, num_stages);
-        return -1;
-    }
-    const double *alpha = lserk_alpha[time_order - 1][num_stages - 1];
-    hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(1), 0, g->stream, g->d_scalars, *time);
+    if (check_single(g, "run_lserk")) return -1;
+    const double *alpha = lserk_coefficients(time_order, num_stages);
+    if (!alpha) return -1;
+    hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(1), 0, g->stream, g->d_scalars,
+                       (const double *)nullptr, *time);
     g->dt_valid = 0;
-    for (int i = 0; i < steps; i++) {
-        int rc = (g->precision == TEAL_FP64)
-                     ? explicit_step(g, g->d64, INFINITY, courant, alpha, num_stages)
-                     : explicit_step(g, g->d32, INFINITY, courant, alpha, num_stages);
-        if (rc) return -1;
-    }
-    if (read_scalars(g)) return -1;
+    int rc = (g->precision == TEAL_FP64) ? run_steps(g, g->d64, steps, courant, alpha, num_stages)
+                                         : run_steps(g, g->d32, steps, courant, alpha, num_stages);
+    if (rc || read_scalars(g)) return -1;
     *time = g->h_scalars->time;
     return 0;
+}
+
+// ------------------------------------------------------------------ batches
+extern "C" int teal_gpu_run_lserk_batch(TealGpu *g, int steps, double *time, double *residual,
+                                        double courant, int time_order, int num_stages)
+{
+    if (check_ready(g)) return -1;
+    if (!time) {
+        teal_gpu_set_error("run_lserk_batch: null time array");
+        return -1;
+    }
+    const double *alpha = lserk_coefficients(time_order, num_stages);
+    if (!alpha) return -1;
+    HIPCALL(hipMemcpyAsync(g->d_stage, time, sizeof(double) * (size_t)g->ninst,
+                           hipMemcpyHostToDevice, g->stream));
+    hipLaunchKernelGGL(k_init_scalars, dim3(g->ninst), dim3(1), 0, g->stream, g->d_scalars,
+                       (const double *)g->d_stage, 0.0);
+    HIPCALL(hipGetLastError());
+    g->dt_valid = 0;
+    int rc = (g->precision == TEAL_FP64) ? run_steps(g, g->d64, steps, courant, alpha, num_stages)
+                                         : run_steps(g, g->d32, steps, courant, alpha, num_stages);
+    if (rc || read_scalars(g)) return -1;
+    for (int i = 0; i < g->ninst; i++) {
+        time[i] = g->h_scalars[i].time;
+        if (residual) memcpy(&residual[(size_t)i * 5], g->h_scalars[i].residual, sizeof(double) * 5);
+    }
+    return 0;
+}
+
+// instance i's host rows <-> its rows of the union's host layout
+static void batch_rows(const TealGpu *g, int i, double *uni, double *inst, int to_union)
+{
+    const int *r = &g->inst_rows[(size_t)6 * i];
+    int local = 0;
+    for (int grp = 0; grp < 3; grp++) {
+        double *u = &uni[(size_t)r[2 * grp] * 5];
+        double *m = &inst[(size_t)local * 5];
+        size_t bytes = sizeof(double) * 5 * (size_t)r[(2 * grp) + 1];
+        if (to_union) {
+            memcpy(u, m, bytes);
+        }
+        else {
+            memcpy(m, u, bytes);
+        }
+        local += r[(2 * grp) + 1];
+    }
+}
+
+static int check_batch(TealGpu *g, const void *arrays, const char *who)
+{
+    if (check_ready(g)) return -1;
+    if (g->inst_rows.empty() || !arrays) {
+        teal_gpu_set_error("%s: not a batch handle, or null arrays", who);
+        return -1;
+    }
+    return 0;
+}
+
+extern "C" int teal_gpu_upload_batch(TealGpu *g, const double *const *primitive)
+{
+    if (check_batch(g, primitive, "upload_batch")) return -1;
+    std::vector<double> uni((size_t)5 * g->mesh->cells.num, 0.0);
+    for (int i = 0; i < g->ninst; i++) {
+        batch_rows(g, i, uni.data(), const_cast<double *>(primitive[i]), 1);
+    }
+    return teal_gpu_upload(g, uni.data());
+}
+
+extern "C" int teal_gpu_download_batch(TealGpu *g, double *const *primitive)
+{
+    if (check_batch(g, primitive, "download_batch")) return -1;
+    std::vector<double> uni((size_t)5 * g->mesh->cells.num, 0.0);
+    if (teal_gpu_download(g, uni.data())) return -1;
+    for (int i = 0; i < g->ninst; i++) {
+        batch_rows(g, i, uni.data(), primitive[i], 0);
+    }
+    return 0;
+}
+
+extern "C" const Mesh *teal_gpu_mesh(TealGpu *g) { return g ? g->mesh : nullptr; }
+
+extern "C" int teal_gpu_batch_size(TealGpu *g)
+{
+    if (!g) {
+        teal_gpu_set_error("null TealGpu");
+        return -1;
+    }
+    return g->ninst;
+}
+
+extern "C" int teal_gpu_set_graphs(TealGpu *g, int enable)
+{
+    if (!g) {
+        teal_gpu_set_error("null TealGpu");
+        return -1;
+    }
+    g->graphs = enable != 0;
+    if (!g->graphs) {
+        drop_graph(g);
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------ autotuning
+// the state an explicit run changes, saved around trial runs
+template <typename S>
+struct Snapshot {
+    S *P = nullptr;
+    double *U0 = nullptr;
+    Scalars *sc = nullptr;
+    int u0_valid = 0, dt_valid = 0;
+
+    int save(TealGpu *g, DevData<S> &d)
+    {
+        HIPCALL(hipMalloc((void **)&P, sizeof(S) * 5 * (size_t)g->nc));
+        HIPCALL(hipMalloc((void **)&U0, sizeof(double) * 5 * (size_t)g->ni));
+        HIPCALL(hipMalloc((void **)&sc, sizeof(Scalars) * (size_t)g->ninst));
+        HIPCALL(hipMemcpyAsync(P, d.d_P, sizeof(S) * 5 * (size_t)g->nc, hipMemcpyDeviceToDevice,
+                               g->stream));
+        HIPCALL(hipMemcpyAsync(U0, g->d_U0, sizeof(double) * 5 * (size_t)g->ni,
+                               hipMemcpyDeviceToDevice, g->stream));
+        HIPCALL(hipMemcpyAsync(sc, g->d_scalars, sizeof(Scalars) * (size_t)g->ninst,
+                               hipMemcpyDeviceToDevice, g->stream));
+        u0_valid = g->u0_valid;
+        dt_valid = g->dt_valid;
+        return 0;
+    }
+    int restore(TealGpu *g, DevData<S> &d)
+    {
+        HIPCALL(hipMemcpyAsync(d.d_P, P, sizeof(S) * 5 * (size_t)g->nc, hipMemcpyDeviceToDevice,
+                               g->stream));
+        HIPCALL(hipMemcpyAsync(g->d_U0, U0, sizeof(double) * 5 * (size_t)g->ni,
+                               hipMemcpyDeviceToDevice, g->stream));
+        HIPCALL(hipMemcpyAsync(g->d_scalars, sc, sizeof(Scalars) * (size_t)g->ninst,
+                               hipMemcpyDeviceToDevice, g->stream));
+        HIPCALL(hipStreamSynchronize(g->stream));
+        g->u0_valid = u0_valid;
+        g->dt_valid = dt_valid;
+        return 0;
+    }
+    ~Snapshot()
+    {
+        (void)hipFree(P);
+        (void)hipFree(U0);
+        (void)hipFree(sc);
+    }
+};
+
+// times `steps` explicit steps (after one untimed step) in each launch shape the handle
+// supports -- fused stage kernel or face/accumulate pair, eager or graph replay -- keeps the
+// fastest, and restores the state; ms[4] receives the per-step times (NaN: not available)
+template <typename S>
+static int autotune(TealGpu *g, DevData<S> &d, int steps, double courant, const double *alpha,
+                    int num_stages, double *ms)
+{
+    const bool can_fuse = d.block_lds() > 0 && d.block_lds() <= (size_t)g->lds_max;
+    Snapshot<S> snap;
+    if (snap.save(g, d)) return -1;
+    const int graphs0 = g->graphs;
+    hipEvent_t t0, t1;
+    HIPCALL(hipEventCreate(&t0));
+    HIPCALL(hipEventCreate(&t1));
+    double best = INFINITY;
+    int best_shape = -1;
+    int rc = 0;
+    for (int shape = 0; shape < 4 && rc == 0; shape++) {
+        const bool fused = shape < 2;
+        ms[shape] = NAN;
+        if (fused && !can_fuse) continue;
+        d.fused = fused;
+        g->graphs = shape & 1;
+        drop_graph(g);
+        rc = snap.restore(g, d) || run_steps(g, d, 1, courant, alpha, num_stages) ||
+             (g->graphs && step_graph(g, d, courant, alpha, num_stages));
+        if (rc) break;
+        (void)hipEventRecord(t0, g->stream);
+        rc = run_steps(g, d, steps, courant, alpha, num_stages);
+        (void)hipEventRecord(t1, g->stream);
+        float t = 0;
+        if (rc || hipEventSynchronize(t1) != hipSuccess || hipEventElapsedTime(&t, t0, t1) != hipSuccess) {
+            rc = -1;
+            break;
+        }
+        ms[shape] = t / steps;
+        if (ms[shape] < best) {
+            best = ms[shape];
+            best_shape = shape;
+        }
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    drop_graph(g);
+    if (rc == 0 && best_shape >= 0) {
+        d.fused = best_shape < 2;
+        g->graphs = best_shape & 1;
+    }
+    else {
+        d.fused = can_fuse;
+        g->graphs = graphs0;
+    }
+    if (snap.restore(g, d)) return -1;
+    return rc;
+}
+
+extern "C" int teal_gpu_autotune(TealGpu *g, int steps, double courant, int time_order,
+                                 int num_stages, double *ms)
+{
+    if (check_ready(g)) return -1;
+    const double *alpha = lserk_coefficients(time_order, num_stages);
+    if (!alpha) return -1;
+    if (steps < 2) {
+        teal_gpu_set_error("autotune: at least 2 timed steps (%d)", steps);
+        return -1;
+    }
+    double tmp[4];
+    double *out = ms ? ms : tmp;
+    return (g->precision == TEAL_FP64) ? autotune(g, g->d64, steps, courant, alpha, num_stages, out)
+                                       : autotune(g, g->d32, steps, courant, alpha, num_stages, out);
 }
 
 // ------------------------------------------------------------------ timing

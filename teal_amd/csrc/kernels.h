@@ -402,7 +402,8 @@ struct StageArgs {
     const double *U0;     // MODE 1/2
     double *U0_next;      // MODE 2: next step's U0 (may alias U0)
     double alpha;         // stage coefficient alpha[i+1]
-    Scalars *scalars;     // step, dt reduction
+    Scalars *scalars;     // step, dt reduction: one record per instance
+    const int *blk_inst;  // instance of each block of rows (null: one instance)
     double *partials;     // MODE 2: [num_blocks][5] residual partial sums
     double *dtpart;       // MODE 2: [num_blocks] minima of the local time step
     int keep_u;           // MODE 2: next U0 = U (mixed) instead of prim2cons(P_new) (FP64)
@@ -428,7 +429,7 @@ __device__ inline void stage_epilogue(const Topo &t, const Geom<S> &g, const Phy
             for (int v = 0; v < 5; v++) a.R[(v * ni) + c] = r[v];
         }
         else {
-            double step = a.scalars->step;
+            double step = a.scalars[a.blk_inst ? a.blk_inst[blk] : 0].step;
             for (int v = 0; v < 5; v++) unew[v] = a.U0[(v * ni) + c] + ((r[v] * a.alpha) * step);
             cons2prim<double>(pnew, unew, ph.gamma);
             for (int v = 0; v < 5; v++) Pout[((long)c * 5) + v] = (S)pnew[v];
@@ -658,7 +659,10 @@ __global__ void k_to_device(const double *__restrict__ in, S *__restrict__ out, 
     if (i >= num) return;
     long src = perm ? perm[i] : i;
     for (int v = 0; v < width; v++) {
-        S x = (S)in[(src * in_stride) + v];
+        // a padding row of a batch (no host row) holds a resting state, rho = p = 1, in the
+        // primitive records (row-major, 5 wide), and zeros in every other array
+        S x = (src >= 0) ? (S)in[(src * in_stride) + v]
+                         : (S)((!SOA && width == 5 && (v == 0 || v == 4)) ? 1 : 0);
         if (SOA) {
             out[((long)v * num) + i] = x;
         }
@@ -675,6 +679,7 @@ __global__ void k_to_host(const S *__restrict__ in, double *__restrict__ out, in
     int i = (blockIdx.x * BLOCK) + threadIdx.x;
     if (i >= num) return;
     long dst = perm ? perm[i] : i;
+    if (dst < 0) return;  // padding row of a batch
     for (int v = 0; v < width; v++) {
         S x = SOA ? in[((long)v * num) + i] : in[((long)i * in_stride) + v];
         out[(dst * out_stride) + v] = (double)x;

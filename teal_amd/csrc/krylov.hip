@@ -666,6 +666,10 @@ extern "C" double teal_gpu_implicit_euler(TealGpu *g, double *time, double *resi
         teal_gpu_set_error("implicit euler: TealGpu is not configured");
         return NAN;
     }
+    if (g->ninst > 1) {
+        teal_gpu_set_error("implicit euler: not available on a batch handle");
+        return NAN;
+    }
     if (!(newton_tol > 0) || !(krylov_tol > 0) || krylov_dim <= 0) {  // src/simulation/set.c:117-125
         teal_gpu_set_error("implicit euler: invalid newton tolerance (%g), krylov tolerance (%g) or "
                            "krylov dimension (%d)", newton_tol, krylov_tol, krylov_dim);
