@@ -246,6 +246,128 @@ def tet_workload(args, n=96, seed=2024):
         "runs": out}), flush=True)
 
 
+def tube_states(rng, count):
+    """Seeded left/right primitive states of `count` shock tubes: rho, p in [0.1, 1], u in
+    [-1, 1], v = w = 0."""
+    lr = np.zeros((count, 2, 5))
+    lr[:, :, 0] = rng.uniform(0.1, 1, (count, 2))
+    lr[:, :, 1] = rng.uniform(-1, 1, (count, 2))
+    lr[:, :, 4] = rng.uniform(0.1, 1, (count, 2))
+    return lr
+
+
+def tube_primitive(a, lr):
+    nc, ni = a["num_cells"], a["num_inner"]
+    P = np.where((a["center"][:nc, 0] < 0.5)[:, None], lr[0], lr[1])
+    P[ni:] = 0
+    return P
+
+
+def batch_workload(args, seed=2026):
+    """Many independent Euler Riemann shock tubes: `--batch` instances on [0,1]^3 Cartesian
+    meshes of n^3 cells, n drawn log-uniformly from [16, 64] (seeded), periodic in y and z,
+    supersonic outflow at x = 0 and 1, seeded left/right states split at x = 0.5; HLLC,
+    LSQ + minmod, LSERK order 2 / 3 stages, CFL 0.5, each instance with its own time step.
+    Times the batch (one handle, every launch covering all instances, autotuned launch shape)
+    against the same instances advanced one at a time (one handle each), FP64, and checks
+    that both give bit-identical states; the mixed batch is timed beside them."""
+    lib = teal_amd.lib()
+    rng = np.random.default_rng(seed)
+    sizes = np.rint(16 * 4 ** rng.uniform(0, 1, args.batch)).astype(int)
+    states = tube_states(rng, args.batch)
+    t0 = time.perf_counter()
+    meshes, arrays, prims = [], [], []
+    for i, n in enumerate(sizes):
+        m = teal_amd.mesh_box((n, n, n), periodic=(0, 1, 1))
+        meshes.append(m)
+        arrays.append(_abi.mesh_arrays(m))
+        prims.append(tube_primitive(arrays[-1], states[i]))
+    cells = int(sum(a["num_inner"] for a in arrays))
+    print(f"# batch: {args.batch} tubes, {cells} cells (n = {sizes.min()}..{sizes.max()}), meshes "
+          f"built in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    bcs = {"left": ("supersonic outflow", None), "right": ("supersonic outflow", None)}
+    cfg = dict(space_order=2, flux="hllc", viscous=0, limiter="minmod", navierstokes=0,
+               gamma=GAMMA, bcs=bcs)
+    steps = args.steps
+
+    # one at a time: each instance on its own handle, eager launches
+    single_s = 0.0
+    single_P = []
+    for i, m in enumerate(meshes):
+        g = teal_amd.Gpu(m, teal_amd.FP64)
+        g.configure(**cfg)
+        g.upload(prims[i])
+        t = C.c_double(0.0)
+        t1 = time.perf_counter()
+        teal_amd.check(lib.teal_gpu_run_lserk(g.h, steps, C.byref(t), CFL, 2, 3), "run_lserk")
+        single_s += time.perf_counter() - t1
+        single_P.append(g.download()[:arrays[i]["num_inner"]])
+        g.close()
+    print(f"# one at a time (fp64): {single_s:.2f} s, {cells * steps / single_s:.4g} cell-updates/s",
+          file=sys.stderr, flush=True)
+
+    runs = {}
+    for name, prec in (("fp64", teal_amd.FP64), ("mixed", teal_amd.MIXED)):
+        b = teal_amd.Batch(meshes, prec)
+        b.configure(**cfg)
+        b.upload_batch(prims)
+        ms = b.autotune(max(2, args.tune_steps), CFL)
+        shapes = ("fused", "fused+graph", "split", "split+graph")
+        best = shapes[int(np.nanargmin(ms))]
+        times = np.zeros(args.batch)
+        if args.warmup:
+            b.run_lserk_batch(args.warmup, times, CFL)
+        b.upload_batch(prims)
+        times = np.zeros(args.batch)
+        t1 = time.perf_counter()
+        times, _ = b.run_lserk_batch(steps, times, CFL)  # ends with a sync
+        secs = time.perf_counter() - t1
+        r = {"seconds": secs, "cell_updates_per_s": cells * steps / secs,
+             "ms_per_step": 1e3 * secs / steps, "autotune_ms_per_step": dict(zip(shapes, ms.tolist())),
+             "shape": best, "layout": b.info()}
+        if name == "fp64":
+            got = b.download_batch()
+            r["equal_to_one_at_a_time"] = bool(all(
+                np.array_equal(got[i][:a["num_inner"]], single_P[i]) for i, a in enumerate(arrays)))
+        # per-kernel times of the stage kernel over a short eager run (events disable graphs)
+        b.upload_batch(prims)
+        teal_amd.check(lib.teal_gpu_timing(b.h, 3 * args.kernel_steps + 3), "timing")
+        b.run_lserk_batch(args.kernel_steps, np.zeros(args.batch), CFL)
+        nl = C.c_int(0)
+        for k, kern in enumerate(("grad", "face", "stage")):
+            r[kern + "_ms"] = lib.teal_gpu_timing_ms(b.h, k, C.byref(nl))
+        lib.teal_gpu_timing(b.h, 0)
+        runs[name] = r
+        b.close()
+        print(f"# batch {name}: {r['ms_per_step']:.3f} ms/step, {r['cell_updates_per_s']:.4g} "
+              f"cell-updates/s, shape {best}, autotune {np.round(ms, 4).tolist()} ms/step; grad "
+              f"{r['grad_ms']:.3f} ms, stage {r['stage_ms']:.3f} ms", file=sys.stderr, flush=True)
+
+    fp = runs["fp64"]
+    u = teal_amd.lib().teal_mesh_union((C.c_void_p * len(meshes))(*meshes), len(meshes))
+    ua = _abi.mesh_arrays(u)
+    model = byte_model(ua["num_inner"], ua["num_cells"], ua["num_faces"], mixed=False, viscous=False)
+    lib.mesh_destroy(u)
+    fused = fp["layout"]["fused"] == 1 and fp["shape"].startswith("fused")
+    kern = "stage" if fused else "accum"
+    achieved = model[kern] / (fp["stage_ms"] * 1e-3) / 1e9 if fp["stage_ms"] > 0 else None
+    print(json.dumps({
+        "metric": "RK cell-updates/s, batched Euler shock tubes vs one-at-a-time launches",
+        "value": fp["cell_updates_per_s"], "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": fp["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.batch} Euler shock tubes, n^3 cells with n log-uniform in "
+                               f"[16, 64] ({cells} cells), HLLC, LSQ + minmod, LSERK 2/3, CFL 0.5",
+                   "cells": cells, "instances": args.batch, "seed": seed},
+        "one_at_a_time": {"seconds": single_s, "cell_updates_per_s": cells * steps / single_s},
+        "speedup_vs_one_at_a_time": fp["cell_updates_per_s"] / (cells * steps / single_s),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                     "kernel": kern},
+        "runs": runs}), flush=True)
+
+
 def channel_workload(args, n=64, seed=5):
     """Navier-Stokes implicit Euler with Jacobian-free Newton-GMRES (NewtonKrylov {0.1, 0.3,
     30}) in a channel [0,2] x [-1,1] x [0,1]: periodic in x and z, no-slip walls in y with the
@@ -315,7 +437,8 @@ def channel_workload(args, n=64, seed=5):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 1000; 500 for the batch workload)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--accuracy-steps", type=int, default=None)
@@ -323,13 +446,21 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE/"
                          "WRITE_SIZE against a kernel of known traffic)")
-    ap.add_argument("--workload", choices=["tgv", "tet", "channel"], default="tgv",
+    ap.add_argument("--batch", type=int, default=256, help="batch workload: instances")
+    ap.add_argument("--tune-steps", type=int, default=10, help="batch workload: autotuner steps")
+    ap.add_argument("--kernel-steps", type=int, default=20,
+                    help="batch workload: steps of the per-kernel timing run")
+    ap.add_argument("--workload", choices=["tgv", "tet", "channel", "batch"], default="tgv",
                     help="tgv: the NS Taylor-Green vortex (the headline metric); tet: Euler "
                          "Mach-2 bump on the jittered tetrahedral mesh (Roe and AUSMD); channel: "
-                         "NS implicit Euler (Newton-GMRES) in a 64^3 stretched-wall channel")
+                         "NS implicit Euler (Newton-GMRES) in a 64^3 stretched-wall channel; "
+                         "batch: 256 Euler shock tubes of 16^3..64^3 cells, batched vs one at "
+                         "a time")
     ap.add_argument("--courant", type=float, default=50.0, help="channel workload CFL")
     ap.add_argument("--channel-n", type=int, default=64, help="channel workload cells per side")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 500 if args.workload == "batch" else 1000
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this library drives one GPU per process; runs on several GPUs are "
               "not part of it", file=sys.stderr)
@@ -351,6 +482,9 @@ def main():
         return
     if args.workload == "channel":
         channel_workload(args, n=args.channel_n)
+        return
+    if args.workload == "batch":
+        batch_workload(args)
         return
     n = args.n
     mesh = teal_amd.mesh_box((n, n, n), (0.0, 0.0, 0.0), (2 * np.pi,) * 3, (1, 1, 1))
