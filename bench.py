@@ -154,34 +154,38 @@ def accuracy(g64, gmx, P0, steps, volume):
             "l2_max": float(l2.max()), "l2_relative_max": float((l2 / scale).max())}
 
 
-def cpu_baseline(n, budget_s=20.0):
-    """The reference's own lserk (oracle/_ref, built from the reference sources) -- or, if it
-    is absent, the oracle restatement -- on the same TGV mesh, one core, a bounded number
-    of steps."""
+def _cpu_stepper(case):
+    """(kind, mesh arrays, primitive view, step function) of the reference's own lserk
+    (oracle/_ref, built from the reference sources) on `case`, or of the oracle restatement
+    when that build is absent."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
 
-    case = ol.Case((n, n, n), lo=(0, 0, 0), hi=(2 * np.pi,) * 3, kind="navierstokes", mu=1 / RE,
-                   prandtl=PR)
     om = ol.oracle_mesh(case)
     arrays = _abi.mesh_arrays(om)
-    nc, ni = arrays["num_cells"], arrays["num_inner"]
-    P0 = tgv_primitive(arrays["center"][:nc])
-    P0[ni:] = 0
-    kind = "reference"
+    nc = arrays["num_cells"]
     try:
         lib = ol.ref()
         e = ol.ref_eqns(case, om)
-        prim = lib.ref_eqns_primitive
+        kind, prim = "reference", lib.ref_eqns_primitive
         step = lambda t, r: lib.ref_lserk(e, C.byref(t), r, 1e300, CFL, 2, 3)
     except OSError:
-        kind = "port"
         lib = ol.oracle()
         e = ol.oracle_eqns(case, om)
-        prim = lib.oracle_eqns_primitive
+        kind, prim = "port", lib.oracle_eqns_primitive
         step = lambda t, r: lib.oracle_lserk(e, C.byref(t), r, 1e300, CFL, 2, 3)
-    p = np.ctypeslib.as_array(prim(e), shape=(nc * 5,))
-    p[:] = P0.ravel()
+    return kind, arrays, np.ctypeslib.as_array(prim(e), shape=(nc * 5,)), step
+
+
+def _tgv_case(n, nz=None, z0=0.0, z1=2 * np.pi):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+
+    return ol.Case((n, n, nz or n), lo=(0, 0, z0), hi=(2 * np.pi, 2 * np.pi, z1),
+                   kind="navierstokes", mu=1 / RE, prandtl=PR)
+
+
+def _step_for(step, budget_s, max_steps):
     t = C.c_double(0.0)
     r = (C.c_double * 5)()
     done = 0
@@ -189,61 +193,69 @@ def cpu_baseline(n, budget_s=20.0):
     while True:
         step(t, r)
         done += 1
-        if time.perf_counter() - t0 >= budget_s or done >= 50:
+        if time.perf_counter() - t0 >= budget_s or done >= max_steps:
             break
-    secs = time.perf_counter() - t0
-    return {"value": ni * done / secs, "unit": "cell-updates/s", "cores": 1, "kind": kind,
-            "sample": f"{done} LSERK step(s) of NS-TGV {n}^3 ({ni} cells), single thread, "
-                      f"{secs:.1f} s"}
+    return done, time.perf_counter() - t0
 
 
-def tet_workload(args, n=96, seed=2024):
-    """Euler on n^3 hexes split into 6 tets each, interior nodes jittered by +-0.2 h (seeded),
-    uniform Mach 2 flow (rho = 1, p = 1/gamma, u = 2) plus a Gaussian density bump of
-    amplitude 0.1 (seeded centre, sigma 0.1), supersonic inflow at x = 0 and supersonic
-    outflow elsewhere; LSQ + Barth-Jespersen, LSERK order 2 / 3 stages, CFL 0.5.
-    Prints one JSON line with FP64 and mixed cell-updates/s for Roe and AUSMD."""
-    lib = teal_amd.lib()
-    t0 = time.perf_counter()
-    mesh = lib.mesh_create_tets(n, 0.2, seed)
-    lib.mesh_generate(mesh)
-    a = _abi.mesh_arrays(mesh)
-    nc, ni, nf = a["num_cells"], a["num_inner"], a["num_faces"]
-    print(f"# tet mesh: {ni} cells, {nf} faces, built in {time.perf_counter() - t0:.1f} s",
-          file=sys.stderr, flush=True)
-    free = (1.0, 2.0, 0.0, 0.0, 1.0 / GAMMA)
-    x = a["center"][:nc]
-    x0 = np.random.default_rng(seed).uniform(0.3, 0.7, 3)
-    P0 = np.tile(np.array(free), (nc, 1))
-    P0[:, 0] += 0.1 * np.exp(-((x - x0) ** 2).sum(axis=1) / (2 * 0.1 ** 2))
+def cpu_worker(spec):
+    """Child process of the multi-core baseline: slab k of K of the n^3 TGV (z in
+    [2 pi k/K, 2 pi (k+1)/K], periodic over the slab in place of the halo exchange -- the
+    compute of one MPI rank of a K-rank slab decomposition), stepped from wall-clock `start`
+    for `budget` seconds; prints one JSON line."""
+    n, k, K, budget, start = spec.split(",")
+    n, k, K, budget, start = int(n), int(k), int(K), float(budget), float(start)
+    h = 2 * np.pi / K
+    kind, a, p, step = _cpu_stepper(_tgv_case(n, n // K, k * h, (k + 1) * h))
+    nc, ni = a["num_cells"], a["num_inner"]
+    P0 = tgv_primitive(a["center"][:nc])
     P0[ni:] = 0
-    bcs = {"left": ("supersonic inflow", free)}
-    bcs.update({s: ("supersonic outflow", None) for s in ("right", "bottom", "top", "back", "front")})
-    out = {}
-    for flux in ("roe", "ausmd"):
-        for name, prec in (("fp64", teal_amd.FP64), ("mixed", teal_amd.MIXED)):
-            g = teal_amd.Gpu(mesh, prec)
-            g.configure(space_order=2, flux=flux, viscous=0, limiter="minmod", navierstokes=0,
-                        gamma=GAMMA, bcs=bcs)
-            r = timed_run(g, P0, args.steps, args.warmup, time_kernels=True)
-            g.close()
-            ups = ni * args.steps / r["seconds"]
-            out[f"{flux}_{name}"] = {"cell_updates_per_s": ups, "ms_per_step": 1e3 * r["seconds"] / args.steps,
-                                     "grad_ms": r["grad_ms"], "stage_ms": r["stage_ms"]}
-            print(f"# tet {flux} {name}: {1e3 * r['seconds'] / args.steps:.3f} ms/step, {ups:.4g} "
-                  f"cell-updates/s; grad {r['grad_ms']:.3f} ms, stage {r['stage_ms']:.3f} ms",
-                  file=sys.stderr, flush=True)
-    best = out["roe_mixed"]
-    print(json.dumps({
-        "metric": "RK cell-updates/s, Euler on a jittered tet mesh, mixed and FP64 GPU",
-        "value": best["cell_updates_per_s"], "unit": "cell-updates/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": best["ms_per_step"],
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 flux/gradient/geometry + f64 update (mixed)", "data": "synthetic",
-        "config": {"workload": f"Euler Mach-2 bump, {n}^3 hexes x 6 jittered tets ({ni} cells), "
-                               "Roe, LSQ + Barth-Jespersen, LSERK 2/3, CFL 0.5",
-                   "cells": ni, "faces": nf},
-        "runs": out}), flush=True)
+    p[:] = P0.ravel()
+    time.sleep(max(0.0, start - time.time()))
+    done, secs = _step_for(step, budget, 1000)
+    print(json.dumps({"kind": kind, "cells": int(ni), "steps": done, "seconds": secs,
+                      "late": max(0.0, time.time() - secs - start)}), flush=True)
+
+
+def cpu_baseline(n, cores=1, budget_s=15.0):
+    """The reference's own lserk on this host's cores, a bounded sample of the NS TGV n^3.
+    cores == 1: one process on the whole mesh.  cores = K > 1: K concurrent processes, one
+    per core, each on one slab of a K-rank slab decomposition (the reference's MPI layout,
+    exchange replaced by periodic copies, so this is an upper bound on what K MPI ranks
+    reach); the aggregate rate is the sum of the processes' rates."""
+    if cores <= 1 or n % cores:
+        kind, a, p, step = _cpu_stepper(_tgv_case(n))
+        nc, ni = a["num_cells"], a["num_inner"]
+        P0 = tgv_primitive(a["center"][:nc])
+        P0[ni:] = 0
+        p[:] = P0.ravel()
+        done, secs = _step_for(step, budget_s, 50)
+        return {"value": ni * done / secs, "unit": "cell-updates/s", "cores": 1, "kind": kind,
+                "sample": f"{done} LSERK step(s) of NS-TGV {n}^3 ({ni} cells), single thread, "
+                          f"{secs:.1f} s"}
+    import subprocess
+
+    start = time.time() + 10.0  # after every child's setup
+    procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker",
+                               f"{n},{k},{cores},{budget_s},{start}"],
+                              stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+             for k in range(cores)]
+    outs = []
+    for p in procs:
+        o, e = p.communicate(timeout=600)
+        if p.returncode:
+            raise RuntimeError(f"cpu worker failed: {e[-500:]}")
+        outs.append(json.loads(o.strip().splitlines()[-1]))
+    rates = [o["cells"] * o["steps"] / o["seconds"] for o in outs]
+    steps = [o["steps"] for o in outs]
+    return {"value": float(sum(rates)), "unit": "cell-updates/s", "cores": cores,
+            "kind": outs[0]["kind"],
+            "sample": f"{cores} concurrent single-thread processes, each stepping one "
+                      f"{n}x{n}x{n // cores} slab of NS-TGV {n}^3 ({outs[0]['cells']} cells, "
+                      f"periodic over the slab in place of the halo exchange) for ~{budget_s:.0f} s "
+                      f"({min(steps)}-{max(steps)} LSERK steps each); value = sum of their rates",
+            "per_process": float(np.mean(rates)),
+            "max_late_start_s": float(max(o["late"] for o in outs))}
 
 
 def tube_states(rng, count):
@@ -265,8 +277,8 @@ def tube_primitive(a, lr):
 
 def batch_workload(args, seed=2026):
     """Many independent Euler Riemann shock tubes: `--batch` instances on [0,1]^3 Cartesian
-    meshes of n^3 cells, n drawn log-uniformly from [16, 64] (seeded), periodic in y and z,
-    supersonic outflow at x = 0 and 1, seeded left/right states split at x = 0.5; HLLC,
+    meshes of n^3 cells, n drawn log-uniformly from [16, 64] (seeded), supersonic outflow at
+    x = 0 and 1, symmetry on the y and z sides, seeded left/right states split at x = 0.5; HLLC,
     LSQ + minmod, LSERK order 2 / 3 stages, CFL 0.5, each instance with its own time step.
     Times the batch (one handle, every launch covering all instances, autotuned launch shape)
     against the same instances advanced one at a time (one handle each), FP64, and checks
@@ -278,7 +290,7 @@ def batch_workload(args, seed=2026):
     t0 = time.perf_counter()
     meshes, arrays, prims = [], [], []
     for i, n in enumerate(sizes):
-        m = teal_amd.mesh_box((n, n, n), periodic=(0, 1, 1))
+        m = teal_amd.mesh_box((n, n, n), periodic=(0, 0, 0))
         meshes.append(m)
         arrays.append(_abi.mesh_arrays(m))
         prims.append(tube_primitive(arrays[-1], states[i]))
@@ -286,6 +298,7 @@ def batch_workload(args, seed=2026):
     print(f"# batch: {args.batch} tubes, {cells} cells (n = {sizes.min()}..{sizes.max()}), meshes "
           f"built in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     bcs = {"left": ("supersonic outflow", None), "right": ("supersonic outflow", None)}
+    bcs.update({side: ("symmetry", None) for side in ("bottom", "top", "back", "front")})
     cfg = dict(space_order=2, flux="hllc", viscous=0, limiter="minmod", navierstokes=0,
                gamma=GAMMA, bcs=bcs)
     steps = args.steps
@@ -358,7 +371,8 @@ def batch_workload(args, seed=2026):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"{args.batch} Euler shock tubes, n^3 cells with n log-uniform in "
-                               f"[16, 64] ({cells} cells), HLLC, LSQ + minmod, LSERK 2/3, CFL 0.5",
+                               f"[16, 64] ({cells} cells), x outflow, y/z symmetry, HLLC, "
+                               "LSQ + minmod, LSERK 2/3, CFL 0.5",
                    "cells": cells, "instances": args.batch, "seed": seed},
         "one_at_a_time": {"seconds": single_s, "cell_updates_per_s": cells * steps / single_s},
         "speedup_vs_one_at_a_time": fp["cell_updates_per_s"] / (cells * steps / single_s),
@@ -443,6 +457,9 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--accuracy-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-cores", type=int, default=16,
+                    help="host cores for the reference CPU baseline (concurrent processes)")
+    ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE/"
                          "WRITE_SIZE against a kernel of known traffic)")
@@ -459,6 +476,9 @@ def main():
     ap.add_argument("--courant", type=float, default=50.0, help="channel workload CFL")
     ap.add_argument("--channel-n", type=int, default=64, help="channel workload cells per side")
     args = ap.parse_args()
+    if args.cpu_worker:
+        cpu_worker(args.cpu_worker)
+        return
     if args.steps is None:
         args.steps = 500 if args.workload == "batch" else 1000
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -494,6 +514,14 @@ def main():
     P0[ni:] = 0
     volume = a["volume"][:ni].copy()
 
+    cpu = None  # first, before this process touches the GPU (it starts child processes)
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(n, max(1, min(args.cpu_cores, os.cpu_count() or 1)))
+        except Exception as exc:  # reported, not fatal
+            cpu = {"value": None, "error": str(exc)}
+        print(f"# cpu baseline: {cpu}", file=sys.stderr, flush=True)
+
     results = {}
     gpus = {}
     for name, prec in (("fp64", teal_amd.FP64), ("mixed", teal_amd.MIXED)):
@@ -524,13 +552,6 @@ def main():
           f"(relative {err['l2_relative_max']:.3g})", file=sys.stderr, flush=True)
     for g in gpus.values():
         g.close()
-
-    cpu = None
-    if not args.no_cpu:
-        try:
-            cpu = cpu_baseline(n)
-        except Exception as exc:  # reported, not fatal
-            cpu = {"value": None, "error": str(exc)}
 
     mx, fp = results["mixed"], results["fp64"]
     KEYS = ["cell_updates_per_s", "ms_per_step", "hbm_fraction_step", "bytes_per_cell_update",

@@ -15,13 +15,14 @@ from teal_amd import _abi
 
 pytestmark = pytest.mark.gpu
 
-BCS = [("left", "supersonic outflow", None), ("right", "supersonic outflow", None)]
+BCS = [("left", "supersonic outflow", None), ("right", "supersonic outflow", None)] + [
+    (side, "symmetry", None) for side in ("bottom", "top", "back", "front")]
 # sizes chosen so no instance fills whole 256-cell blocks (padding rows in every instance)
 SIZES = [(5, 3, 3), (7, 4, 4), (9, 3, 5), (4, 4, 4), (11, 6, 5)]
 
 
 def tube_case(n):
-    return Case(n, periodic=(0, 1, 1), flux="hllc", limiter="minmod", bcs=BCS)
+    return Case(n, periodic=(0, 0, 0), flux="hllc", limiter="minmod", bcs=BCS)
 
 
 def tube_state(center, ni, nc, seed):
