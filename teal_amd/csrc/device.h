@@ -54,7 +54,13 @@ struct DevData {
     fv::Geom<S> geom() const;
     fv::BlockTopo block_topo() const;
     fv::Geom<S> block_geom() const;  // cell geometry + face geometry in block-list order
-    size_t block_lds() const { return sizeof(S) * 10 * (size_t)max_block_faces; }
+    // dynamic LDS of the fused stage kernel: the face records of the largest block list
+    // (values per face as FaceRec: 10 for FP64 with viscous and convective fluxes, else 5)
+    size_t block_lds(int visc = 1, int conv = 1) const
+    {
+        const int nv = (sizeof(S) == 8 && visc && conv) ? 10 : 5;
+        return sizeof(S) * nv * (size_t)max_block_faces;
+    }
 };
 
 // kernel launchers of one precision; instantiated in device.hip (double) and mixed.hip (float)

@@ -528,26 +528,31 @@ int DevData<S>::build_blocks(const std::vector<int> &inc, int ni, const std::vec
     const int nj = (int)list.size();
     const size_t cap = (size_t)std::max(nj, 1);
     num_block_faces = nj;
-    // FP64: the centre form (48 fewer bytes per face) where it reproduces the stored geometry;
-    // FP32 keeps the offsets rounded from FP64 (a difference of FP32 centres loses digits)
+    // FP64: the centre form (48 fewer bytes per face) where it reproduces the stored geometry.
+    // FP32 keeps the offsets rounded from FP64 (a difference of FP32 centres loses digits) but
+    // derives the correction from them and the tangents from the normal (40 fewer bytes).
     centre_form = sizeof(S) == 8 && centres_reproduce(flr, crow, fcen, ccen, offl, offr, corr);
+    const bool fp64 = sizeof(S) == 8;
+    const int brows = fp64 ? 9 : 3;
     std::vector<int2> lflr(cap);
     std::vector<int> lcrow(cap);
-    std::vector<double> larea(cap), lbasis(9 * cap), loffl, loffr, lcorr, lfcen;
+    std::vector<double> larea(cap), lbasis(brows * cap), loffl, loffr, lcorr, lfcen;
     if (centre_form) {
         lfcen.assign(3 * cap, 0.0);
     }
     else {
         loffl.assign(3 * cap, 0.0);
         loffr.assign(3 * cap, 0.0);
-        lcorr.assign(4 * cap, 0.0);
+        if (fp64) {
+            lcorr.assign(4 * cap, 0.0);
+        }
     }
     for (int j = 0; j < nj; j++) {
         int f = list[j];
         lflr[j] = flr[f];
         lcrow[j] = crow[f];
         larea[j] = area[f];
-        for (int k = 0; k < 9; k++) lbasis[((size_t)k * nj) + j] = basis[((size_t)k * nf) + f];
+        for (int k = 0; k < brows; k++) lbasis[((size_t)k * nj) + j] = basis[((size_t)k * nf) + f];
         if (centre_form) {
             for (int k = 0; k < 3; k++) lfcen[((size_t)k * nj) + j] = fcen[((size_t)k * nf) + f];
             continue;
@@ -556,7 +561,9 @@ int DevData<S>::build_blocks(const std::vector<int> &inc, int ni, const std::vec
             loffl[((size_t)k * nj) + j] = offl[((size_t)k * nf) + f];
             loffr[((size_t)k * nj) + j] = offr[((size_t)k * nf) + f];
         }
-        for (int k = 0; k < 4; k++) lcorr[((size_t)k * nj) + j] = corr[((size_t)k * nf) + f];
+        if (fp64) {
+            for (int k = 0; k < 4; k++) lcorr[((size_t)k * nj) + j] = corr[((size_t)k * nf) + f];
+        }
     }
     if (upload(&d_boff, off) || upload(&d_bslot, slot) || upload(&d_bflr, lflr) ||
         upload_conv(&d_barea, larea) || upload_conv(&d_bbasis, lbasis)) {
@@ -567,7 +574,8 @@ int DevData<S>::build_blocks(const std::vector<int> &inc, int ni, const std::vec
             return -1;
         }
     }
-    else if (upload_conv(&d_boffl, loffl) || upload_conv(&d_boffr, loffr) || upload_conv(&d_bcorr, lcorr)) {
+    else if (upload_conv(&d_boffl, loffl) || upload_conv(&d_boffr, loffr) ||
+             (fp64 && upload_conv(&d_bcorr, lcorr))) {
         return -1;
     }
     return 0;
@@ -583,7 +591,8 @@ template <typename S>
 Geom<S> DevData<S>::block_geom() const
 {
     return Geom<S>{d_vol,    d_proj,   d_eps2,   d_w,      d_o,    d_barea,
-                   d_bbasis, d_boffl,  d_boffr,  d_bcorr,  d_bfcen, d_ccen, d_bcrow};
+                   d_bbasis, d_boffl,  d_boffr,  d_bcorr,  d_bfcen, d_ccen, d_bcrow,
+                   sizeof(S) == 4};
 }
 
 template <typename S>
@@ -954,7 +963,8 @@ int teal_residual_pass(TealGpu *g, DevData<S> &d, int mode, double alpha, int la
     if (d.fused) {  // one kernel: face fluxes in LDS, accumulation, stage update into d_P2
         bool timed = g->timing && g->ev_flux.begin(g->stream);
         int rc = Kern<S>::flux_block(g->stream, g->cfg.flux, g->cfg.viscous, g->cfg.space_order, kmode,
-                                     d.block_lds(), g->topo(), d.block_topo(), d.block_geom(),
+                                     d.block_lds(g->cfg.viscous, conv), g->topo(), d.block_topo(),
+                                     d.block_geom(),
                                      g->phys(),
                                      d.d_P, d.d_GP, d.d_P2, a);
         if (timed) {

@@ -71,10 +71,11 @@ struct Geom {
     const S *__restrict__ w;      // [3][W][ni]
     const S *__restrict__ o;      // [3][W][ni]
     const S *__restrict__ area;   // [nf]
-    const S *__restrict__ basis;  // [9][nf]
+    const S *__restrict__ basis;  // [9][nf], or [3][nf] (the normal alone) when `tangents`
     const S *__restrict__ offl;   // [3][nf]
     const S *__restrict__ offr;   // [3][nf]
-    const S *__restrict__ corr;   // [4][nf] unit xyz, norm
+    const S *__restrict__ corr;   // [4][nf] unit xyz, norm; null with offl set: derived from
+                                  // the offsets (cr - cl = offl - offr)
     // centre form (offl, offr, corr null): the face offsets and corrections are recomputed
     // from the face centre and the cell centres with the expressions that produced them
     // (src/mesh/generate.c:754-777), so they are bit-identical
@@ -82,7 +83,36 @@ struct Geom {
     const S *__restrict__ ccen;   // [num_cells][3] cell centres (device rows)
     const int *__restrict__ crow; // [nf] centre row of the right cell (the ghost row of a
                                   // periodic face, whose flr.y is its source cell)
+    int tangents = 0;             // 1: basis holds the normal; tangents are rebuilt from it
 };
+
+// Face tangents from the unit normal n with the construction of src/mesh/generate.c:420-435
+// (FP32 face records keep only n).  The stored basis may hold both tangents negated (the
+// reference flips x and y of faces whose normal pointed into the left cell, 437-451); the
+// convective fluxes are invariant under that rotation of the tangent plane.
+template <typename S>
+__device__ inline void face_tangents(S *b9)
+{
+    const S nx = b9[0], ny = b9[1], nz = b9[2];
+    const S nqz = sqrt((nx * nx) + (ny * ny));
+    const S nqy = sqrt((nx * nx) + (nz * nz));
+    if (nqz > nqy) {
+        b9[3] = -ny / nqz;
+        b9[4] = nx / nqz;
+        b9[5] = 0;
+        b9[6] = (-nx * nz) / nqz;
+        b9[7] = (-ny * nz) / nqz;
+        b9[8] = nqz;
+    }
+    else {
+        b9[3] = (-nx * ny) / nqy;
+        b9[4] = nqy;
+        b9[5] = (-ny * nz) / nqy;
+        b9[6] = -nz / nqy;
+        b9[7] = 0;
+        b9[8] = nx / nqy;
+    }
+}
 
 struct Phys {
     double gamma, mu, prandtl;
@@ -147,6 +177,18 @@ struct Acc<float> {
     }
     __device__ void sub(int i, float x) { add(i, -x); }
     __device__ double get(int i) const { return (double)hi[i] + (double)lo[i]; }
+};
+
+// Values kept per face between the face phase and the accumulation.  FP64 keeps the viscous
+// and convective contributions apart, for the reference's order (every viscous face of a
+// cell, then every convective one: derivative.c:116-153, 191-224).  FP32 merges them into one
+// record (Fc - Fv) A per face, summed into the cell with one compensated addition.
+template <typename S, int VISC, int CONV>
+struct FaceRec {
+    static constexpr bool MERGE = sizeof(S) == 4 && VISC && CONV;
+    static constexpr int NV = (VISC && CONV && !MERGE) ? 10 : 5;  // values per face
+    static constexpr int CONV_OFF = (VISC && !MERGE) ? 5 : 0;      // first convective value
+    static constexpr int AVISC = MERGE ? 0 : VISC;                  // accumulation passes
 };
 
 template <typename S>
@@ -284,8 +326,15 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
     if (g.offl) {
         ol = v3(g.offl[(0 * nf) + f], g.offl[(1 * nf) + f], g.offl[(2 * nf) + f]);
         orr = v3(g.offr[(0 * nf) + f], g.offr[(1 * nf) + f], g.offr[(2 * nf) + f]);
-        cu = v3(g.corr[(0 * nf) + f], g.corr[(1 * nf) + f], g.corr[(2 * nf) + f]);
-        cn = g.corr[(3 * nf) + f];
+        if (g.corr) {
+            cu = v3(g.corr[(0 * nf) + f], g.corr[(1 * nf) + f], g.corr[(2 * nf) + f]);
+            cn = g.corr[(3 * nf) + f];
+        }
+        else {
+            V3<S> dl = v3(ol.x - orr.x, ol.y - orr.y, ol.z - orr.z);
+            cn = sqrt(dot(dl, dl));
+            cu = v3(dl.x / cn, dl.y / cn, dl.z / cn);
+        }
     }
     else {
         const long rr = g.crow[f];
@@ -363,7 +412,13 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
             }
         }
         S b9[9];
-        for (int i = 0; i < 9; i++) b9[i] = g.basis[(i * nf) + f];
+        if (g.tangents) {
+            for (int i = 0; i < 3; i++) b9[i] = g.basis[(i * nf) + f];
+            face_tangents(b9);
+        }
+        else {
+            for (int i = 0; i < 9; i++) b9[i] = g.basis[(i * nf) + f];
+        }
         S flx[5];
         convective<FLUX, S>(flx, ql, qr, gamma, m3(b9), err);
         for (int v = 0; v < 5; v++) fc[v] = flx[v] * area;
@@ -376,7 +431,8 @@ __global__ __launch_bounds__(BLOCK) void k_face(Topo t, Geom<S> g, Phys ph, cons
                                                 const S *__restrict__ GP, S *__restrict__ F,
                                                 int *error)
 {
-    constexpr int NF = (VISC && FLUX != FLUX_NONE) ? 10 : 5;
+    using R = FaceRec<S, VISC, FLUX != FLUX_NONE>;
+    constexpr int NF = R::NV;
     const int f = (xcd_block() * BLOCK) + threadIdx.x;
     if (f >= t.nf) {
         return;
@@ -385,11 +441,16 @@ __global__ __launch_bounds__(BLOCK) void k_face(Topo t, Geom<S> g, Phys ph, cons
     int err = 0;
     face_flux<S, FLUX, VISC, ORDER>(t, g, ph, P, GP, f, fv, fc, &err);
     S *out = &F[(long)f * NF];
-    if (VISC) {
-        for (int v = 0; v < 5; v++) out[v] = fv[v];
+    if (R::MERGE) {
+        for (int v = 0; v < 5; v++) out[v] = fc[v] - fv[v];
     }
-    if (FLUX != FLUX_NONE) {
-        for (int v = 0; v < 5; v++) out[(VISC ? 5 : 0) + v] = fc[v];
+    else {
+        if (VISC) {
+            for (int v = 0; v < 5; v++) out[v] = fv[v];
+        }
+        if (FLUX != FLUX_NONE) {
+            for (int v = 0; v < 5; v++) out[R::CONV_OFF + v] = fc[v];
+        }
     }
     if (FLUX == FLUX_GODUNOV && err) {
         atomicMax(error, err);
@@ -539,15 +600,15 @@ template <typename S, int VISC, int CONV, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_accum(Topo t, Geom<S> g, Phys ph, const S *__restrict__ F,
                                                  S *__restrict__ Pout, StageArgs a)
 {
-    constexpr int NF = (VISC && CONV) ? 10 : 5;
+    using R = FaceRec<S, VISC, CONV>;
     const int blk = xcd_block();
     const int c = (blk * BLOCK) + threadIdx.x;
     const bool active = c < t.ni;
     Acc<S> acc;
     acc.zero();
     if (active) {
-        accumulate<S, VISC, CONV>(t, t.inc, c, acc, [&](int e, int part, S *x) {
-            const S *rec = &F[((long)(e >> 2) * NF) + ((part && VISC) ? 5 : 0)];
+        accumulate<S, R::AVISC, CONV>(t, t.inc, c, acc, [&](int e, int part, S *x) {
+            const S *rec = &F[((long)(e >> 2) * R::NV) + (part ? R::CONV_OFF : 0)];
             for (int v = 0; v < 5; v++) x[v] = rec[v];
         });
     }
@@ -575,8 +636,9 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux_block(Topo t, BlockTopo b
                                                       S *__restrict__ Pout, StageArgs a)
 {
     constexpr int CONV = FLUX != FLUX_NONE;
+    using R = FaceRec<S, VISC, CONV>;
     extern __shared__ unsigned char lds_raw[];
-    S *lds = (S *)lds_raw;  // [10][max_faces]: viscous 0..4, convective 5..9
+    S *lds = (S *)lds_raw;  // [R::NV][max_faces]
     const int mf = bt.max_faces;
     const int blk = xcd_block();
     const int beg = bt.off[blk];
@@ -588,11 +650,16 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux_block(Topo t, BlockTopo b
     for (int j = threadIdx.x; j < num; j += FLUX_THREADS) {
         S fv[5], fc[5];
         face_flux<S, FLUX, VISC, ORDER>(tf, g, ph, P, GP, beg + j, fv, fc, &err);
-        if (VISC) {
-            for (int v = 0; v < 5; v++) lds[(v * mf) + j] = fv[v];
+        if (R::MERGE) {
+            for (int v = 0; v < 5; v++) lds[(v * mf) + j] = fc[v] - fv[v];
         }
-        if (CONV) {
-            for (int v = 0; v < 5; v++) lds[((5 + v) * mf) + j] = fc[v];
+        else {
+            if (VISC) {
+                for (int v = 0; v < 5; v++) lds[(v * mf) + j] = fv[v];
+            }
+            if (CONV) {
+                for (int v = 0; v < 5; v++) lds[((R::CONV_OFF + v) * mf) + j] = fc[v];
+            }
         }
     }
     if (FLUX == FLUX_GODUNOV && err) {
@@ -604,9 +671,9 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux_block(Topo t, BlockTopo b
     Acc<S> acc;
     acc.zero();
     if (active) {
-        accumulate<S, VISC, CONV>(t, bt.slot, c, acc, [&](int e, int part, S *x) {
+        accumulate<S, R::AVISC, CONV>(t, bt.slot, c, acc, [&](int e, int part, S *x) {
             const int j = e >> 1;
-            for (int v = 0; v < 5; v++) x[v] = lds[(((part ? 5 : 0) + v) * mf) + j];
+            for (int v = 0; v < 5; v++) x[v] = lds[(((part ? R::CONV_OFF : 0) + v) * mf) + j];
         });
     }
     stage_epilogue<S, MODE, FLUX_THREADS>(t, g, ph, acc, c, active, blk, Pout, a);
