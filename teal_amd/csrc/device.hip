@@ -337,9 +337,13 @@ int DevData<S>::build(const Mesh *m, TealGpu *gpu)
         return cinv[cell];
     };
 
+    if (nc >= (1 << 30)) {  // neighbour entries pack row << 1 | side into an int
+        teal_gpu_set_error("mesh too large for one device handle (%d rows)", nc);
+        return -1;
+    }
     // incidences: walking faces in reference order keeps each list sorted by reference index
     std::vector<int> cnt(ni, 0);
-    std::vector<int> inc((size_t)W * ni, -1), nbr((size_t)W * ni, 0);
+    std::vector<int> inc((size_t)W * ni, -1), nbr((size_t)W * ni, -1);
     std::vector<double> w((size_t)3 * W * ni, 0.0), o((size_t)3 * W * ni, 0.0);
     for (int f = 0; f < nf; f++) {
         int l = m->faces.cell_idx[f].left;
@@ -357,7 +361,7 @@ int DevData<S>::build(const Mesh *m, TealGpu *gpu)
             }
             size_t idx = ((size_t)k * ni) + c;
             inc[idx] = (finv[f] << 2) | (bnd << 1) | side;
-            nbr[idx] = side ? cinv[l] : resolve(r);
+            nbr[idx] = ((side ? cinv[l] : resolve(r)) << 1) | side;
             Vector wv = side ? m->faces.weight[f].right : m->faces.weight[f].left;
             Vector ov = side ? m->faces.offset[f].right : m->faces.offset[f].left;
             double wa[3] = {wv.x, wv.y, wv.z};

@@ -59,7 +59,8 @@ constexpr int NUM_XCD = 8;
 struct Topo {
     int ni, nc, nf;
     const int *__restrict__ inc;   // [W][ni]
-    const int *__restrict__ nbr;   // [W][ni] neighbour cell (device numbering / ghost row)
+    const int *__restrict__ nbr;   // [W][ni] neighbour row << 1 | side of the cell (-1: none);
+                                   // the gradient kernel reads it instead of inc
     const int2 *__restrict__ flr;  // [nf] left, right (~ghost for boundary faces)
 };
 
@@ -248,12 +249,12 @@ __global__ __launch_bounds__(BLOCK) void k_grad_limit(Topo t, Geom<S> g, const S
     }
 #pragma unroll
     for (int k = 0; k < W; k++) {
-        int e = t.inc[(k * ni) + c];
-        if (e < 0) {
+        const int nb = t.nbr[(k * ni) + c];
+        if (nb < 0) {
             continue;
         }
-        int n = t.nbr[(k * ni) + c];
-        int side = e & 1;
+        const int n = nb >> 1;
+        const int side = nb & 1;
         S wx = g.w[(((0 * W) + k) * ni) + c];
         S wy = g.w[(((1 * W) + k) * ni) + c];
         S wz = g.w[(((2 * W) + k) * ni) + c];
@@ -277,8 +278,7 @@ __global__ __launch_bounds__(BLOCK) void k_grad_limit(Topo t, Geom<S> g, const S
         S eps2 = (LIM == 2) ? g.eps2[c] : S(0);
 #pragma unroll
         for (int k = 0; k < W; k++) {
-            int e = t.inc[(k * ni) + c];
-            if (e < 0) {
+            if (t.nbr[(k * ni) + c] < 0) {
                 continue;
             }
             V3<S> o = v3(g.o[(((0 * W) + k) * ni) + c], g.o[(((1 * W) + k) * ni) + c],
