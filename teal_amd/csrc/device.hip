@@ -73,9 +73,15 @@ __global__ __launch_bounds__(DT_THREADS) void k_dt_min(Scalars *scalars, const d
     const int b0 = blk_off[blockIdx.x];
     const int b1 = blk_off[blockIdx.x + 1];
     double dt = __longlong_as_double(0x7ff0000000000000LL);
-    for (int b = b0 + threadIdx.x; b < b1; b += DT_THREADS) {
-        dt = fmin(dt, dtpart[b]);
+    double d[4] = {dt, dt, dt, dt};  // four loads in flight per thread
+    int b = b0 + threadIdx.x;
+    for (; b + (3 * DT_THREADS) < b1; b += 4 * DT_THREADS) {
+        for (int u = 0; u < 4; u++) d[u] = fmin(d[u], dtpart[b + (u * DT_THREADS)]);
     }
+    for (; b < b1; b += DT_THREADS) {
+        d[0] = fmin(d[0], dtpart[b]);
+    }
+    dt = fmin(fmin(d[0], d[1]), fmin(d[2], d[3]));
     for (int off = 32; off > 0; off >>= 1) dt = fmin(dt, __shfl_xor(dt, off));
     if ((threadIdx.x & 63) == 0) {
         red[threadIdx.x >> 6] = dt;
@@ -95,8 +101,11 @@ __global__ __launch_bounds__(DT_THREADS) void k_dt_min(Scalars *scalars, const d
 // (src/equations/residual.c:8-33), then time += step (advance.c:111).  Workgroup i handles
 // the blocks [blk_off[i], blk_off[i + 1]) of instance i, of total volume volume[i].  Thread t
 // of the STEP_END_THREADS = 5 x 128 sums variable t % 5 of blocks t / 5, t / 5 + 128, ...
-// (coalesced reads), then a fixed tree over the 128 partial sums.
+// (coalesced reads) into STEP_END_SUMS running sums taken round-robin (that many loads in
+// flight), adds those pairwise, then a fixed tree over the 128 partial sums.
 constexpr int STEP_END_THREADS = 640;
+constexpr int STEP_END_SUMS = 8;
+static_assert(STEP_END_THREADS % 5 == 0 && STEP_END_SUMS == 8, "k_step_end layout");
 
 __global__ __launch_bounds__(STEP_END_THREADS) void k_step_end(Scalars *scalars, const double *partials,
                                                                const int *blk_off,
@@ -109,10 +118,18 @@ __global__ __launch_bounds__(STEP_END_THREADS) void k_step_end(Scalars *scalars,
     const long e1 = (long)blk_off[blockIdx.x + 1] * 5;
     const int t = threadIdx.x;
     const int lanes = STEP_END_THREADS / 5;
-    double s = 0;
-    for (long e = e0 + t; e < e1; e += STEP_END_THREADS) {
-        s += partials[e];
+    double acc[STEP_END_SUMS] = {0, 0, 0, 0, 0, 0, 0, 0};
+    long e = e0 + t;
+    for (; e + ((STEP_END_SUMS - 1) * STEP_END_THREADS) < e1; e += STEP_END_SUMS * STEP_END_THREADS) {
+        for (int u = 0; u < STEP_END_SUMS; u++) acc[u] += partials[e + (u * STEP_END_THREADS)];
     }
+    for (int u = 0; e < e1; e += STEP_END_THREADS, u++) {
+        acc[u] += partials[e];
+    }
+    for (int h = STEP_END_SUMS / 2; h > 0; h >>= 1) {
+        for (int u = 0; u < h; u++) acc[u] += acc[u + h];
+    }
+    double s = acc[0];
     red[t] = s;
     __syncthreads();
     for (int h = lanes / 2; h > 0; h >>= 1) {
