@@ -247,28 +247,33 @@ __global__ __launch_bounds__(BLOCK) void k_grad_limit(Topo t, Geom<S> g, const S
     else {
         for (int i = 0; i < 15; i++) gr[i] = GP[((long)c * 20) + i];
     }
+    // every incidence's loads are issued before any is consumed (absent incidences load the
+    // cell's own row and are masked), so a thread waits on memory twice, not twice per face
+    int nb[W];
+#pragma unroll
+    for (int k = 0; k < W; k++) nb[k] = t.nbr[(k * ni) + c];
+    S pn[W][5], wk[W][3];
 #pragma unroll
     for (int k = 0; k < W; k++) {
-        const int nb = t.nbr[(k * ni) + c];
-        if (nb < 0) {
+        const long n = (nb[k] < 0) ? c : (nb[k] >> 1);
+        load_rec(pn[k], P, n, 5);
+        for (int d = 0; d < 3; d++) wk[k][d] = g.w[(((d * W) + k) * ni) + c];
+    }
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        if (nb[k] < 0) {
             continue;
         }
-        const int n = nb >> 1;
-        const int side = nb & 1;
-        S wx = g.w[(((0 * W) + k) * ni) + c];
-        S wy = g.w[(((1 * W) + k) * ni) + c];
-        S wz = g.w[(((2 * W) + k) * ni) + c];
-        S pn[5];
-        load_rec(pn, P, n, 5);
+        const int side = nb[k] & 1;
         for (int v = 0; v < 5; v++) {
-            mn[v] = tmin(mn[v], pn[v]);
-            mx[v] = tmax(mx[v], pn[v]);
+            mn[v] = tmin(mn[v], pn[k][v]);
+            mx[v] = tmax(mx[v], pn[k][v]);
             if (MODE == 0) {
-                S diff = side ? (pc[v] - pn[v]) : (pn[v] - pc[v]);  // P[right] - P[left]
+                S diff = side ? (pc[v] - pn[k][v]) : (pn[k][v] - pc[v]);  // P[right] - P[left]
                 S s = side ? -diff : diff;
-                gr[(v * 3) + 0] += s * wx;
-                gr[(v * 3) + 1] += s * wy;
-                gr[(v * 3) + 2] += s * wz;
+                gr[(v * 3) + 0] += s * wk[k][0];
+                gr[(v * 3) + 1] += s * wk[k][1];
+                gr[(v * 3) + 2] += s * wk[k][2];
             }
         }
     }
@@ -276,13 +281,17 @@ __global__ __launch_bounds__(BLOCK) void k_grad_limit(Topo t, Geom<S> g, const S
     for (int v = 0; v < 5; v++) ps[v] = 1;
     if (LIM != 0) {
         S eps2 = (LIM == 2) ? g.eps2[c] : S(0);
+        S ok[W][3];
 #pragma unroll
         for (int k = 0; k < W; k++) {
-            if (t.nbr[(k * ni) + c] < 0) {
+            for (int d = 0; d < 3; d++) ok[k][d] = g.o[(((d * W) + k) * ni) + c];
+        }
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            if (nb[k] < 0) {
                 continue;
             }
-            V3<S> o = v3(g.o[(((0 * W) + k) * ni) + c], g.o[(((1 * W) + k) * ni) + c],
-                         g.o[(((2 * W) + k) * ni) + c]);
+            V3<S> o = v3(ok[k][0], ok[k][1], ok[k][2]);
             for (int v = 0; v < 5; v++) {
                 S d2 = dot(v3(gr[(v * 3) + 0], gr[(v * 3) + 1], gr[(v * 3) + 2]), o);
                 if (tabs(d2) <= S(1e-8)) {  // is_close(d2, 0), src/core/utils.c:29-34
