@@ -320,12 +320,11 @@ __global__ __launch_bounds__(BLOCK) void k_grad_limit(Topo t, Geom<S> g, const S
 // Viscous (fv) and convective (fc) contribution F*A of face f.
 template <typename S, int FLUX, int VISC, int ORDER>
 __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph,
-                                 const S *__restrict__ P, const S *__restrict__ GP, int f, S *fv,
-                                 S *fc, int *err)
+                                 const S *__restrict__ P, const S *__restrict__ GP, int f,
+                                 int2 lr, S *fv, S *fc, int *err)
 {
     const int nf = t.nf;
     const S gamma = (S)ph.gamma;
-    int2 lr = t.flr[f];
     const int l = lr.x;
     const bool bnd = lr.y < 0;
     const int r = bnd ? ~lr.y : lr.y;
@@ -448,7 +447,7 @@ __global__ __launch_bounds__(BLOCK) void k_face(Topo t, Geom<S> g, Phys ph, cons
     }
     S fv[5], fc[5];
     int err = 0;
-    face_flux<S, FLUX, VISC, ORDER>(t, g, ph, P, GP, f, fv, fc, &err);
+    face_flux<S, FLUX, VISC, ORDER>(t, g, ph, P, GP, f, t.flr[f], fv, fc, &err);
     S *out = &F[(long)f * NF];
     if (R::MERGE) {
         for (int v = 0; v < 5; v++) out[v] = fc[v] - fv[v];
@@ -656,9 +655,15 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux_block(Topo t, BlockTopo b
     Topo tf = t;  // face arrays in list order
     tf.nf = bt.nj;
     tf.flr = bt.flr;
+    // the next face's cell pair is loaded while this face is computed (its gathers need it)
+    int2 lr_next = (threadIdx.x < num) ? bt.flr[beg + threadIdx.x] : int2{0, 0};
     for (int j = threadIdx.x; j < num; j += FLUX_THREADS) {
+        const int2 lr = lr_next;
+        if (j + FLUX_THREADS < num) {
+            lr_next = bt.flr[beg + j + FLUX_THREADS];
+        }
         S fv[5], fc[5];
-        face_flux<S, FLUX, VISC, ORDER>(tf, g, ph, P, GP, beg + j, fv, fc, &err);
+        face_flux<S, FLUX, VISC, ORDER>(tf, g, ph, P, GP, beg + j, lr, fv, fc, &err);
         if (R::MERGE) {
             for (int v = 0; v < 5; v++) lds[(v * mf) + j] = fc[v] - fv[v];
         }
