@@ -192,6 +192,14 @@ struct FaceRec {
     static constexpr int AVISC = MERGE ? 0 : VISC;                  // accumulation passes
 };
 
+// Geometry streamed once per launch (face records, per-incidence weights and offsets):
+// loaded non-temporally, so it does not displace the gathered states in L2 and MALL.
+template <typename T>
+__device__ inline T ld_stream(const T *p)
+{
+    return __builtin_nontemporal_load(p);
+}
+
 template <typename S>
 __device__ inline void load_rec(S *dst, const S *__restrict__ src, long row, int width)
 {
@@ -257,7 +265,7 @@ __global__ __launch_bounds__(BLOCK) void k_grad_limit(Topo t, Geom<S> g, const S
     for (int k = 0; k < W; k++) {
         const long n = (nb[k] < 0) ? c : (nb[k] >> 1);
         load_rec(pn[k], P, n, 5);
-        for (int d = 0; d < 3; d++) wk[k][d] = g.w[(((d * W) + k) * ni) + c];
+        for (int d = 0; d < 3; d++) wk[k][d] = ld_stream(&g.w[(((d * W) + k) * ni) + c]);
     }
 #pragma unroll
     for (int k = 0; k < W; k++) {
@@ -284,7 +292,7 @@ __global__ __launch_bounds__(BLOCK) void k_grad_limit(Topo t, Geom<S> g, const S
         S ok[W][3];
 #pragma unroll
         for (int k = 0; k < W; k++) {
-            for (int d = 0; d < 3; d++) ok[k][d] = g.o[(((d * W) + k) * ni) + c];
+            for (int d = 0; d < 3; d++) ok[k][d] = ld_stream(&g.o[(((d * W) + k) * ni) + c]);
         }
 #pragma unroll
         for (int k = 0; k < W; k++) {
@@ -332,11 +340,11 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
     V3<S> ol, orr, cu;
     S cn;
     if (g.offl) {
-        ol = v3(g.offl[(0 * nf) + f], g.offl[(1 * nf) + f], g.offl[(2 * nf) + f]);
-        orr = v3(g.offr[(0 * nf) + f], g.offr[(1 * nf) + f], g.offr[(2 * nf) + f]);
+        ol = v3(ld_stream(&g.offl[(0 * nf) + f]), ld_stream(&g.offl[(1 * nf) + f]), ld_stream(&g.offl[(2 * nf) + f]));
+        orr = v3(ld_stream(&g.offr[(0 * nf) + f]), ld_stream(&g.offr[(1 * nf) + f]), ld_stream(&g.offr[(2 * nf) + f]));
         if (g.corr) {
-            cu = v3(g.corr[(0 * nf) + f], g.corr[(1 * nf) + f], g.corr[(2 * nf) + f]);
-            cn = g.corr[(3 * nf) + f];
+            cu = v3(ld_stream(&g.corr[(0 * nf) + f]), ld_stream(&g.corr[(1 * nf) + f]), ld_stream(&g.corr[(2 * nf) + f]));
+            cn = ld_stream(&g.corr[(3 * nf) + f]);
         }
         else {
             V3<S> dl = v3(ol.x - orr.x, ol.y - orr.y, ol.z - orr.z);
@@ -346,7 +354,7 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
     }
     else {
         const long rr = g.crow[f];
-        V3<S> fc = v3(g.fcen[(0 * nf) + f], g.fcen[(1 * nf) + f], g.fcen[(2 * nf) + f]);
+        V3<S> fc = v3(ld_stream(&g.fcen[(0 * nf) + f]), ld_stream(&g.fcen[(1 * nf) + f]), ld_stream(&g.fcen[(2 * nf) + f]));
         V3<S> cl = v3(g.ccen[((long)l * 3) + 0], g.ccen[((long)l * 3) + 1], g.ccen[((long)l * 3) + 2]);
         V3<S> cr = v3(g.ccen[(rr * 3) + 0], g.ccen[(rr * 3) + 1], g.ccen[(rr * 3) + 2]);
         ol = v3(fc.x - cl.x, fc.y - cl.y, fc.z - cl.z);
@@ -365,7 +373,7 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
             load_rec(gr, GP, r, 20);
         }
     }
-    const S area = g.area[f];
+    const S area = ld_stream(&g.area[f]);
     if (VISC) {
         const S mu = (S)ph.mu;
         const S prl = (S)ph.prandtl;
@@ -389,7 +397,7 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
             gm[v].y -= a3.y;
             gm[v].z -= a3.z;
         }
-        V3<S> nrm = v3(g.basis[(0 * nf) + f], g.basis[(1 * nf) + f], g.basis[(2 * nf) + f]);
+        V3<S> nrm = v3(ld_stream(&g.basis[(0 * nf) + f]), ld_stream(&g.basis[(1 * nf) + f]), ld_stream(&g.basis[(2 * nf) + f]));
         S fl[5];
         viscous_central<S>(fl, pm, gm, gamma, mu, prl, nrm);
         for (int v = 0; v < 5; v++) fv[v] = fl[v] * area;
@@ -421,11 +429,11 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
         }
         S b9[9];
         if (g.tangents) {
-            for (int i = 0; i < 3; i++) b9[i] = g.basis[(i * nf) + f];
+            for (int i = 0; i < 3; i++) b9[i] = ld_stream(&g.basis[(i * nf) + f]);
             face_tangents(b9);
         }
         else {
-            for (int i = 0; i < 9; i++) b9[i] = g.basis[(i * nf) + f];
+            for (int i = 0; i < 9; i++) b9[i] = ld_stream(&g.basis[(i * nf) + f]);
         }
         S flx[5];
         convective<FLUX, S>(flx, ql, qr, gamma, m3(b9), err);
