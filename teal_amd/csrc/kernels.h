@@ -192,8 +192,9 @@ struct FaceRec {
     static constexpr int AVISC = MERGE ? 0 : VISC;                  // accumulation passes
 };
 
-// Geometry streamed once per launch (face records, per-incidence weights and offsets):
-// loaded non-temporally, so it does not displace the gathered states in L2 and MALL.
+// Arrays streamed once per launch (face records, per-incidence weights, offsets and
+// indices, cell volumes and projections, U0): loaded non-temporally, so they do not displace
+// the gathered states in L2 and MALL.
 template <typename T>
 __device__ inline T ld_stream(const T *p)
 {
@@ -259,7 +260,7 @@ __global__ __launch_bounds__(BLOCK) void k_grad_limit(Topo t, Geom<S> g, const S
     // cell's own row and are masked), so a thread waits on memory twice, not twice per face
     int nb[W];
 #pragma unroll
-    for (int k = 0; k < W; k++) nb[k] = t.nbr[(k * ni) + c];
+    for (int k = 0; k < W; k++) nb[k] = ld_stream(&t.nbr[(k * ni) + c]);
     S pn[W][5], wk[W][3];
 #pragma unroll
     for (int k = 0; k < W; k++) {
@@ -500,14 +501,14 @@ __device__ inline void stage_epilogue(const Topo &t, const Geom<S> &g, const Phy
     double pnew[5] = {0, 0, 0, 0, 0};
     double unew[5] = {0, 0, 0, 0, 0};
     if (active) {
-        double vol = (double)g.vol[c];
+        double vol = (double)ld_stream(&g.vol[c]);
         for (int v = 0; v < 5; v++) r[v] = -acc.get(v) / vol;
         if (MODE == 0) {
             for (int v = 0; v < 5; v++) a.R[(v * ni) + c] = r[v];
         }
         else {
             double step = a.scalars[a.blk_inst ? a.blk_inst[blk] : 0].step;
-            for (int v = 0; v < 5; v++) unew[v] = a.U0[(v * ni) + c] + ((r[v] * a.alpha) * step);
+            for (int v = 0; v < 5; v++) unew[v] = ld_stream(&a.U0[(v * ni) + c]) + ((r[v] * a.alpha) * step);
             cons2prim<double>(pnew, unew, ph.gamma);
             for (int v = 0; v < 5; v++) Pout[((long)c * 5) + v] = (S)pnew[v];
         }
@@ -519,8 +520,9 @@ __device__ inline void stage_epilogue(const Topo &t, const Geom<S> &g, const Phy
         if (active) {
             double pst[5];  // the stored state, as the next step's timestep() reads it
             for (int v = 0; v < 5; v++) pst[v] = (double)(S)pnew[v];
-            double vol = (double)g.vol[c];
-            V3<double> pj = v3((double)g.proj[c], (double)g.proj[ni + c], (double)g.proj[(2 * ni) + c]);
+            double vol = (double)ld_stream(&g.vol[c]);
+            V3<double> pj = v3((double)ld_stream(&g.proj[c]), (double)ld_stream(&g.proj[ni + c]),
+                                (double)ld_stream(&g.proj[(2 * ni) + c]));
             dt = ph.navierstokes ? ns_dt<double>(pst, ph.gamma, ph.mu, ph.prandtl, vol, pj)
                                  : euler_dt<double>(pst, ph.gamma, vol, pj);
             double u0n[5];
@@ -574,7 +576,7 @@ __device__ inline void accumulate(const Topo &t, const int *__restrict__ slots, 
     if (VISC) {
 #pragma unroll
         for (int k = 0; k < W; k++) {
-            int e = slots[(k * ni) + c];
+            int e = ld_stream(&slots[(k * ni) + c]);
             if (e < 0) {
                 continue;
             }
@@ -593,7 +595,7 @@ __device__ inline void accumulate(const Topo &t, const int *__restrict__ slots, 
     if (CONV) {
 #pragma unroll
         for (int k = 0; k < W; k++) {
-            int e = slots[(k * ni) + c];
+            int e = ld_stream(&slots[(k * ni) + c]);
             if (e < 0) {
                 continue;
             }
@@ -714,8 +716,9 @@ __global__ __launch_bounds__(BLOCK) void k_timestep_u0(Topo t, Geom<S> g, Phys p
     if (c < t.ni) {
         double p[5];
         for (int v = 0; v < 5; v++) p[v] = (double)P[((long)c * 5) + v];
-        double vol = (double)g.vol[c];
-        V3<double> pj = v3((double)g.proj[c], (double)g.proj[t.ni + c], (double)g.proj[(2 * t.ni) + c]);
+        double vol = (double)ld_stream(&g.vol[c]);
+        V3<double> pj = v3((double)ld_stream(&g.proj[c]), (double)ld_stream(&g.proj[t.ni + c]),
+                            (double)ld_stream(&g.proj[(2 * t.ni) + c]));
         dt = ph.navierstokes ? ns_dt<double>(p, ph.gamma, ph.mu, ph.prandtl, vol, pj)
                              : euler_dt<double>(p, ph.gamma, vol, pj);
         if (write_u0) {
