@@ -533,7 +533,7 @@ __device__ inline void stage_epilogue(const Topo &t, const Geom<S> &g, const Phy
                 prim2cons<double>(u0n, pst, ph.gamma);
             }
             for (int v = 0; v < 5; v++) {
-                a.U0_next[(v * ni) + c] = u0n[v];
+                __builtin_nontemporal_store(u0n[v], &a.U0_next[(v * ni) + c]);  // read next step
                 sums[v] = vol * (r[v] * r[v]);
             }
         }
