@@ -41,6 +41,7 @@ struct DevData {
     bool centre_form = false;
     int num_block_faces = 0;
     int max_block_faces = 0;
+    int max_inc = fv::W;  // most incidences of any inner cell
     int fused = 0;  // 1: residual passes use k_flux_block (its face lists fit in LDS and the
                     // environment does not set TEAL_GPU_SPLIT=1)
 

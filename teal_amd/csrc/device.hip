@@ -389,6 +389,7 @@ int DevData<S>::build(const Mesh *m, TealGpu *gpu)
             }
         }
     }
+    max_inc = *std::max_element(cnt.begin(), cnt.end());
     std::vector<double> vol(ni, 1.0), proj((size_t)3 * ni, 0.0);  // padding rows: 1 and 0
     for (int d = 0; d < ni; d++) {
         int c = cperm[d];
@@ -946,9 +947,9 @@ int teal_fp64_mirror(TealGpu *g)
 Topo TealGpu::topo() const
 {
     if (precision == TEAL_FP64) {
-        return Topo{ni, nc, nf, d64.d_inc, d64.d_nbr, d64.d_flr};
+        return Topo{ni, nc, nf, d64.d_inc, d64.d_nbr, d64.d_flr, d64.max_inc};
     }
-    return Topo{ni, nc, nf, d32.d_inc, d32.d_nbr, d32.d_flr};
+    return Topo{ni, nc, nf, d32.d_inc, d32.d_nbr, d32.d_flr, d32.max_inc};
 }
 
 Phys TealGpu::phys() const { return Phys{cfg.gamma, cfg.mu, cfg.prandtl, cfg.navierstokes}; }

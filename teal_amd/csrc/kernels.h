@@ -62,6 +62,7 @@ struct Topo {
     const int *__restrict__ nbr;   // [W][ni] neighbour row << 1 | side of the cell (-1: none);
                                    // the gradient kernel reads it instead of inc
     const int2 *__restrict__ flr;  // [nf] left, right (~ghost for boundary faces)
+    int wmax = W;                  // most incidences of any inner cell (4: tetrahedra)
 };
 
 template <typename S>
@@ -237,7 +238,8 @@ __global__ __launch_bounds__(BLOCK) void k_boundary(int num, int nf, const int *
 
 // ---------------------------------------------------------------- gradient + limiter
 // MODE 0: compute the LSQ gradient then limit; MODE 1: limiter only on the gradient in GP.
-template <typename S, int LIM, int MODE>
+// WK: incidence rows visited (W, or 4 on meshes of tetrahedra, whose rows 4 and 5 are empty).
+template <typename S, int LIM, int MODE, int WK = W>
 __global__ __launch_bounds__(BLOCK) void k_grad_limit(Topo t, Geom<S> g, const S *__restrict__ P,
                                                       S *__restrict__ GP)
 {
@@ -258,18 +260,18 @@ __global__ __launch_bounds__(BLOCK) void k_grad_limit(Topo t, Geom<S> g, const S
     }
     // every incidence's loads are issued before any is consumed (absent incidences load the
     // cell's own row and are masked), so a thread waits on memory twice, not twice per face
-    int nb[W];
+    int nb[WK];
 #pragma unroll
-    for (int k = 0; k < W; k++) nb[k] = ld_stream(&t.nbr[(k * ni) + c]);
-    S pn[W][5], wk[W][3];
+    for (int k = 0; k < WK; k++) nb[k] = ld_stream(&t.nbr[(k * ni) + c]);
+    S pn[WK][5], wk[WK][3];
 #pragma unroll
-    for (int k = 0; k < W; k++) {
+    for (int k = 0; k < WK; k++) {
         const long n = (nb[k] < 0) ? c : (nb[k] >> 1);
         load_rec(pn[k], P, n, 5);
         for (int d = 0; d < 3; d++) wk[k][d] = ld_stream(&g.w[(((d * W) + k) * ni) + c]);
     }
 #pragma unroll
-    for (int k = 0; k < W; k++) {
+    for (int k = 0; k < WK; k++) {
         if (nb[k] < 0) {
             continue;
         }
@@ -290,13 +292,13 @@ __global__ __launch_bounds__(BLOCK) void k_grad_limit(Topo t, Geom<S> g, const S
     for (int v = 0; v < 5; v++) ps[v] = 1;
     if (LIM != 0) {
         S eps2 = (LIM == 2) ? g.eps2[c] : S(0);
-        S ok[W][3];
+        S ok[WK][3];
 #pragma unroll
-        for (int k = 0; k < W; k++) {
+        for (int k = 0; k < WK; k++) {
             for (int d = 0; d < 3; d++) ok[k][d] = ld_stream(&g.o[(((d * W) + k) * ni) + c]);
         }
 #pragma unroll
-        for (int k = 0; k < W; k++) {
+        for (int k = 0; k < WK; k++) {
             if (nb[k] < 0) {
                 continue;
             }

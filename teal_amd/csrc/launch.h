@@ -146,21 +146,33 @@ int Kern<S>::flux_block(hipStream_t s, int flux, int visc, int order, int mode, 
     return 0;
 }
 
-template <typename S>
-void Kern<S>::grad_limit(hipStream_t s, int lim, int mode, fv::Topo t, fv::Geom<S> g, const S *P,
-                         S *GP)
+template <typename S, int WK>
+static void grad_limit_wk(hipStream_t s, int lim, int mode, fv::Topo t, fv::Geom<S> g, const S *P,
+                          S *GP)
 {
     dim3 grid(fv::grid_of(t.ni)), block(fv::BLOCK);
     if (mode == 0) {
         switch (lim) {
-            case 0: hipLaunchKernelGGL((fv::k_grad_limit<S, 0, 0>), grid, block, 0, s, t, g, P, GP); return;
-            case 1: hipLaunchKernelGGL((fv::k_grad_limit<S, 1, 0>), grid, block, 0, s, t, g, P, GP); return;
-            default: hipLaunchKernelGGL((fv::k_grad_limit<S, 2, 0>), grid, block, 0, s, t, g, P, GP); return;
+            case 0: hipLaunchKernelGGL((fv::k_grad_limit<S, 0, 0, WK>), grid, block, 0, s, t, g, P, GP); return;
+            case 1: hipLaunchKernelGGL((fv::k_grad_limit<S, 1, 0, WK>), grid, block, 0, s, t, g, P, GP); return;
+            default: hipLaunchKernelGGL((fv::k_grad_limit<S, 2, 0, WK>), grid, block, 0, s, t, g, P, GP); return;
         }
     }
     switch (lim) {
-        case 1: hipLaunchKernelGGL((fv::k_grad_limit<S, 1, 1>), grid, block, 0, s, t, g, P, GP); return;
-        default: hipLaunchKernelGGL((fv::k_grad_limit<S, 2, 1>), grid, block, 0, s, t, g, P, GP); return;
+        case 1: hipLaunchKernelGGL((fv::k_grad_limit<S, 1, 1, WK>), grid, block, 0, s, t, g, P, GP); return;
+        default: hipLaunchKernelGGL((fv::k_grad_limit<S, 2, 1, WK>), grid, block, 0, s, t, g, P, GP); return;
+    }
+}
+
+template <typename S>
+void Kern<S>::grad_limit(hipStream_t s, int lim, int mode, fv::Topo t, fv::Geom<S> g, const S *P,
+                         S *GP)
+{
+    if (t.wmax <= 4) {
+        grad_limit_wk<S, 4>(s, lim, mode, t, g, P, GP);
+    }
+    else {
+        grad_limit_wk<S, fv::W>(s, lim, mode, t, g, P, GP);
     }
 }
 
