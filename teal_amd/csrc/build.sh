@@ -2,7 +2,8 @@
 # Build libteal_amd.so for gfx950 in place (teal_amd/libteal_amd.so).
 #   FP64 kernels: -ffp-contract=off (bit-compatible with the reference)
 #   FP32 kernels: contraction on, FP32 division and square root to within a few ulp (the
-#                 hardware reciprocal and square-root estimates) instead of correctly rounded
+#                 hardware reciprocal and square-root estimates) instead of correctly rounded,
+#                 FP32 denormals flushed to zero (FP64 arithmetic keeps IEEE denormals)
 # Packed FP32 instructions are disabled (-packed-fp32-ops) on every HIP line.
 set -euo pipefail
 HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
@@ -22,7 +23,8 @@ build_hip() {  # src flags...
     /opt/rocm/bin/hipcc "${HIPFLAGS[@]}" "$@" -c "$HERE/$src" -o "$OBJ/${src%.*}.o"
 }
 build_hip device.hip -ffp-contract=off & pids+=($!)
-build_hip mixed.hip -ffp-contract=fast -fno-hip-fp32-correctly-rounded-divide-sqrt & pids+=($!)
+build_hip mixed.hip -ffp-contract=fast -fno-hip-fp32-correctly-rounded-divide-sqrt \
+    -fgpu-flush-denormals-to-zero & pids+=($!)
 build_hip krylov.hip -ffp-contract=off & pids+=($!)
 build_hip plugins.hip -ffp-contract=off & pids+=($!)
 for src in core.c sync.c mesh_create.c mesh_generate.c mesh_util.c mesh_union.c equations.c simulation.c; do
