@@ -1,5 +1,5 @@
 """Mixed path (FP32 flux / gradient / geometry, compensated FP32 accumulation, FP64 update)
-against the FP64 oracle.  Its error is stated, not zero: the bounds below are the ones this
+against the FP64 oracle, on both the fused stage kernel and the face + accumulation kernels.  Its error is stated, not zero: the bounds below are the ones this
 path is held to (relative to the largest magnitude of each quantity)."""
 import ctypes as C
 
@@ -9,7 +9,7 @@ import pytest
 import teal_amd
 from cases import LARGE_CASES, RESIDUAL_CASES, initial_state
 from oracle_lib import dp, oracle
-from test_gpu_residual import setup_pair
+from test_gpu_residual import path, setup_pair  # noqa: F401 (path: fused / split fixture)
 
 pytestmark = pytest.mark.gpu
 
@@ -24,8 +24,8 @@ CASES = RESIDUAL_CASES[:3] + RESIDUAL_CASES[3:4] + RESIDUAL_CASES[12:15] + LARGE
 
 
 @pytest.mark.parametrize("name,case,kind", CASES, ids=[c[0] for c in CASES])
-def test_mixed_derivative_error(name, case, kind):
-    print(f"mixed {name}", flush=True)
+def test_mixed_derivative_error(name, case, kind, path):
+    print(f"mixed {name} {path}", flush=True)
     om, oe, gpu, arrays, P = setup_pair(case, kind, precision=teal_amd.MIXED)
     Po = P.copy()
     Ro = np.zeros((arrays["num_inner"], 5))
@@ -39,8 +39,8 @@ def test_mixed_derivative_error(name, case, kind):
 
 
 @pytest.mark.parametrize("name,case,kind", CASES, ids=[c[0] for c in CASES])
-def test_mixed_lserk_error(name, case, kind):
-    print(f"mixed lserk {name}", flush=True)
+def test_mixed_lserk_error(name, case, kind, path):
+    print(f"mixed lserk {name} {path}", flush=True)
     om, oe, gpu, arrays, P = setup_pair(case, kind, precision=teal_amd.MIXED)
     ni, nc = arrays["num_inner"], arrays["num_cells"]
     po = np.ctypeslib.as_array(oracle().oracle_eqns_primitive(oe), shape=(nc * 5,))
