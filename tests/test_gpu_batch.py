@@ -91,6 +91,45 @@ def test_batch_fp64_matches_each_instance_alone():
     batch.close()
 
 
+def test_batch_of_tets_and_hexes():
+    """Instances of different cell types in one batch (jittered tetrahedra next to hexahedra):
+    each matches the oracle stepping it alone, bit for bit, over 8 steps."""
+    steps = 8
+    print(f"batch tets + hexes {steps} steps", flush=True)
+    lib = teal_amd.lib()
+    o = oracle()
+    o.oracle_mesh_copy_created.restype = C.c_void_p
+    o.oracle_mesh_copy_created.argtypes = [C.c_void_p]
+    meshes, states, oracles = [], [], []
+    for i, (kind, n) in enumerate([("tets", 4), ("hex", (5, 3, 3)), ("tets", 3), ("hex", (4, 4, 6))]):
+        case = tube_case(n if kind == "hex" else (n, n, n))
+        if kind == "tets":
+            pm = lib.mesh_create_tets(n, 0.2, 100 + i)
+            om = o.oracle_mesh_copy_created(pm)
+            o.oracle_mesh_generate(om)
+            lib.mesh_generate(pm)
+        else:
+            pm = teal_amd.mesh_box(n, periodic=case.periodic)
+            om = oracle_mesh(case)
+        oe = oracle_eqns(case, om)
+        a = _abi.mesh_arrays(om)
+        meshes.append(pm)
+        states.append(tube_state(a["center"], a["num_inner"], a["num_cells"], 50 + i))
+        oracles.append((om, oe, a["num_cells"], a["num_inner"]))
+    batch = teal_amd.Batch(meshes, teal_amd.FP64)
+    batch.configure(space_order=2, flux="hllc", limiter="minmod",
+                    bcs={ent: (name, ctx) for ent, name, ctx in BCS})
+    batch.upload_batch(states)
+    times, res = batch.run_lserk_batch(steps, np.zeros(len(meshes)), 0.5)
+    got = batch.download_batch()
+    for i, ((om, oe, nc, ni), P) in enumerate(zip(oracles, states)):
+        Po, t_o, r_o = oracle_run(om, oe, nc, P, steps)
+        assert times[i] == t_o, (i, times[i], t_o)
+        np.testing.assert_allclose(res[i], r_o, rtol=1e-12, atol=1e-300)
+        np.testing.assert_array_equal(got[i][:ni], Po[:ni])
+    batch.close()
+
+
 def test_batch_union_upload_layout():
     """teal_gpu_upload of the union layout and teal_gpu_upload_batch agree."""
     print("batch union upload", flush=True)
