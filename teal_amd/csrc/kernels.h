@@ -504,14 +504,25 @@ __device__ inline void stage_epilogue(const Topo &t, const Geom<S> &g, const Phy
     double unew[5] = {0, 0, 0, 0, 0};
     if (active) {
         double vol = (double)ld_stream(&g.vol[c]);
-        for (int v = 0; v < 5; v++) r[v] = -acc.get(v) / vol;
+        if constexpr (sizeof(S) == 8) {  // the reference's divisions (derivative.c:253-258)
+            for (int v = 0; v < 5; v++) r[v] = -acc.get(v) / vol;
+        }
+        else {  // FP32 path: one reciprocal, FP64 products
+            const double iv = 1.0 / vol;
+            for (int v = 0; v < 5; v++) r[v] = -acc.get(v) * iv;
+        }
         if (MODE == 0) {
             for (int v = 0; v < 5; v++) a.R[(v * ni) + c] = r[v];
         }
         else {
             double step = a.scalars[a.blk_inst ? a.blk_inst[blk] : 0].step;
             for (int v = 0; v < 5; v++) unew[v] = ld_stream(&a.U0[(v * ni) + c]) + ((r[v] * a.alpha) * step);
-            cons2prim<double>(pnew, unew, ph.gamma);
+            if constexpr (sizeof(S) == 8) {
+                cons2prim<double>(pnew, unew, ph.gamma);
+            }
+            else {
+                cons2prim_recip<double>(pnew, unew, ph.gamma);
+            }
             for (int v = 0; v < 5; v++) Pout[((long)c * 5) + v] = (S)pnew[v];
         }
     }

@@ -872,6 +872,19 @@ template <typename T> TEAL_HD void cons2prim(T *p, const T *u, T gamma)
     p[4] = tmax(T(1e-8), (gamma - 1) * (u[4] - (dot(v3(u[1], u[2], u[3]), v) / 2)));
 }
 
+// cons2prim with one reciprocal of the density in place of three divisions (the FP32 path's
+// FP64 update, which is not held to the reference's rounding)
+template <typename T> TEAL_HD void cons2prim_recip(T *p, const T *u, T gamma)
+{
+    p[0] = tmax(T(1e-8), u[0]);
+    const T ir = T(1) / p[0];
+    V3<T> v = v3(u[1] * ir, u[2] * ir, u[3] * ir);
+    p[1] = v.x;
+    p[2] = v.y;
+    p[3] = v.z;
+    p[4] = tmax(T(1e-8), (gamma - 1) * (u[4] - (dot(v3(u[1], u[2], u[3]), v) / 2)));
+}
+
 template <typename T> TEAL_HD void prim2cons(T *u, const T *p, T gamma)
 {
     u[0] = p[0];
