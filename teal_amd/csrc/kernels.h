@@ -8,14 +8,19 @@
 //                         states, periodic ghosts are never stored (they resolve to their
 //                         source cell)
 //   GP   [num_inner][20]  LSQ gradient (var*3 + xyz, 15 values) then limiter factor psi (5)
-//   F    [num_faces][NF]  face contributions F*A: viscous (5, when present) then convective (5)
+//   F    [num_faces][NF]  face contributions F*A: FP64 viscous (5, when present) then
+//                         convective (5); FP32 one merged record (Fc - Fv)*A (FaceRec)
 //   U0   [5][num_inner]   FP64 conserved state at the start of the RK step
 //   incidences, slot-major [W][num_inner]: inc = face<<2 | boundary<<1 | side (side 1: the cell
 //   is the face's right cell), slots sorted by the REFERENCE face index, so each cell sums
 //   its face contributions in the order the reference scatters them
-//   (src/equations/derivative.c:116-224); per-incidence LSQ weight and face offset [3][W][ni]
+//   (src/equations/derivative.c:116-224); nbr = neighbour row << 1 | side for the gradient
+//   kernel; per-incidence LSQ weight and face offset [3][W][ni]
 //   face geometry, structure of arrays [..][num_faces]: area, basis (9), offset l/r (3+3),
-//   correction (4); face cells [num_faces][2] (right < 0: ~ghost of a boundary face)
+//   correction (4); face cells [num_faces][2] (right < 0: ~ghost of a boundary face).  The
+//   fused kernel reads a copy in block-list order: FP64 in centre form (face and cell centres
+//   instead of offsets and corrections) where that reproduces the stored values; FP32 with
+//   the normal alone (tangents rebuilt) and the correction derived from the offsets
 // Records (P, GP, F) are arrays of structures because they are gathered through cell and
 // face indices: one record is one or two cache lines instead of one line per component.
 //
@@ -25,10 +30,11 @@
 //                one thread per inner cell
 //   k_flux_block one workgroup per block of 256 device cells: the viscous and convective flux
 //                of every face touching the block (derivative.c:91-227) into LDS, then per
-//                cell the viscous contributions in face order, the convective ones, R = -R/V
-//                (derivative.c:229-267) and, in RK mode, the stage update + cons2prim
-//                (src/simulation/advance.c:101-109); the last stage also produces the next
-//                step's U0, the dt minimum and residual partial sums
+//                cell the viscous contributions in face order, the convective ones (FP32:
+//                the merged records, compensated), R = -R/V (derivative.c:229-267) and, in
+//                RK mode, the stage update + cons2prim (src/simulation/advance.c:101-109);
+//                the last stage also produces the next step's U0, the dt minimum and
+//                residual partial sums
 //   k_face + k_accum  the same work as two kernels through a face flux array in HBM, used
 //                when a block's face list does not fit in LDS
 #pragma once
