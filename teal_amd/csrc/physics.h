@@ -479,23 +479,31 @@ template <typename T> TEAL_HD void flux_hllc(T *flux, const T *lp, const T *rp, 
     Local<T> r = to_local(rp, gamma, b);
     T ssl, ssr;
     roe_signal_speeds(l, r, gamma, &ssl, &ssr);
+    // the branches of convective.c:279-346 pick a side k and a formula; the side is selected
+    // first so that lanes of one wave taking different sides evaluate one formula, not both
     Cons<T> f;
-    if (ssl >= 0) {
-        f = physical_flux(l);
-    }
-    else if (ssr <= 0) {
-        f = physical_flux(r);
-    }
-    else {
+    const bool super = (ssl >= 0) || (ssr <= 0);
+    T ss = 0, ssk = 0, fk = 0;
+    bool left = ssl >= 0;
+    if (!super) {
         T fl = l.rho * (ssl - l.u.x);
         T fr = r.rho * (ssr - r.u.x);
-        T ss = (r.p - l.p + (fl * l.u.x) - (fr * r.u.x)) / (fl - fr);
-        if (ssl <= 0 && 0 <= ss) {
-            f = contact_flux(l, ssl, fl, ss);
-        }
-        else {
-            f = contact_flux(r, ssr, fr, ss);
-        }
+        ss = (r.p - l.p + (fl * l.u.x) - (fr * r.u.x)) / (fl - fr);
+        left = (ssl <= 0 && 0 <= ss);
+        ssk = left ? ssl : ssr;
+        fk = left ? fl : fr;
+    }
+    Local<T> k;
+    k.rho = left ? l.rho : r.rho;
+    k.u = v3(left ? l.u.x : r.u.x, left ? l.u.y : r.u.y, left ? l.u.z : r.u.z);
+    k.p = left ? l.p : r.p;
+    k.m = v3(left ? l.m.x : r.m.x, left ? l.m.y : r.m.y, left ? l.m.z : r.m.z);
+    k.e = left ? l.e : r.e;
+    if (super) {
+        f = physical_flux(k);
+    }
+    else {
+        f = contact_flux(k, ssk, fk, ss);
     }
     to_global(flux, f, b);
 }
