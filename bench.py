@@ -1,13 +1,16 @@
 #!/usr/bin/env python3
 """Benchmark: RK cell-updates/s of the Navier-Stokes Taylor-Green vortex on a periodic 128^3
 hex mesh (2,097,152 cells), FP64 GPU path vs mixed FP32/FP64 GPU path, with the mixed path's
-error against FP64, the flux kernel's HBM roofline fraction, and the reference CPU path timed
-on this host.
+error against FP64, the stage kernel's HBM roofline fraction, and the reference CPU path
+timed on this host's cores (first, before the GPU is touched).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--cpu-cores 16]
+    python bench.py --workload tet|channel|batch   (the other configurations)
 
 One step = one 3-stage LSERK step (src/simulation/advance.c:47-117) of every cell; the state
-stays resident in HBM for the whole timed region (no host transfers inside it).
+stays resident in HBM for the whole timed region (no host transfers inside it).  Each path
+first lets the autotuner pick its launch shape; per-kernel durations come from a separate
+run with HIP events around every launch.
 """
 import argparse
 import ctypes as C
