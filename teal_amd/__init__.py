@@ -162,6 +162,15 @@ class Gpu:
         check(self.lib.teal_gpu_derivative(self.h, dp(P), dp(R)), "derivative")
         return R, P
 
+    def residual(self, R):
+        """sqrt(sum V R^2 / sum V) per variable of a derivative in host layout: 5 values, or
+        one row of 5 per instance on a batch handle."""
+        R = np.ascontiguousarray(R, dtype=np.float64)
+        n = self.lib.teal_gpu_batch_size(self.h)
+        out = np.zeros((n, 5))
+        check(self.lib.teal_gpu_residual(self.h, dp(R), dp(out)), "residual")
+        return out[0] if n == 1 else out
+
     def upload(self, P):
         P = np.ascontiguousarray(P, dtype=np.float64)
         check(self.lib.teal_gpu_upload(self.h, dp(P)), "upload")

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import teal_amd
-from oracle_lib import Case, oracle, oracle_eqns, oracle_mesh
+from oracle_lib import Case, dp, oracle, oracle_eqns, oracle_mesh
 from teal_amd import _abi
 
 pytestmark = pytest.mark.gpu
@@ -127,6 +127,30 @@ def test_batch_of_tets_and_hexes():
         assert times[i] == t_o, (i, times[i], t_o)
         np.testing.assert_allclose(res[i], r_o, rtol=1e-12, atol=1e-300)
         np.testing.assert_array_equal(got[i][:ni], Po[:ni])
+    batch.close()
+
+
+def test_batch_derivative_and_residuals():
+    """The per-call routines on a batch handle: the derivative of the union (host layout of
+    teal_mesh_union) equals each instance's oracle derivative bit for bit, and
+    teal_gpu_residual gives one residual norm per instance (rtol 1e-12)."""
+    print("batch derivative + residuals", flush=True)
+    _, _, states, oracles, batch = build(SIZES)
+    batch.upload_batch(states)
+    union = batch.download()  # the union's host layout
+    R, _ = batch.derivative(union)
+    res = batch.residual(R)
+    assert res.shape == (len(SIZES), 5)
+    row = 0
+    for i, ((om, oe, nc, ni), P) in enumerate(zip(oracles, states)):
+        Ro = np.zeros((ni, 5))
+        o = oracle()
+        o.oracle_derivative(oe, dp(P.copy()), dp(Ro))
+        np.testing.assert_array_equal(R[row:row + ni], Ro)
+        ro = np.zeros(5)
+        o.oracle_residual(oe, dp(Ro), dp(ro))
+        np.testing.assert_allclose(res[i], ro, rtol=1e-12, atol=1e-300)
+        row += ni
     batch.close()
 
 
