@@ -965,7 +965,9 @@ int teal_residual_pass(TealGpu *g, DevData<S> &d, int mode, double alpha, int la
     }
     if (g->cfg.space_order == 2) {
         bool timed = g->timing && g->ev_grad.begin(g->stream);
-        Kern<S>::grad_limit(g->stream, g->cfg.limiter, 0, g->topo(), d.geom(), d.d_P, d.d_GP);
+        // inviscid passes keep only the limited gradient (15 values, in rows of 16)
+        Kern<S>::grad_limit(g->stream, g->cfg.limiter, g->cfg.viscous ? 0 : 2, g->topo(), d.geom(),
+                            d.d_P, d.d_GP);
         if (timed) {
             g->ev_grad.end(g->stream);
         }

@@ -243,7 +243,10 @@ __global__ __launch_bounds__(BLOCK) void k_boundary(int num, int nf, const int *
 }
 
 // ---------------------------------------------------------------- gradient + limiter
-// MODE 0: compute the LSQ gradient then limit; MODE 1: limiter only on the gradient in GP.
+// MODE 0: compute the LSQ gradient then limit, GP rows [ni][20] = gradient, psi;
+// MODE 1: limiter only on the gradient in GP; MODE 2 (inviscid stages, which use the limited
+// gradient alone): compute and limit, GP rows [ni][16] = gradient * psi (limiter.c:61-63)
+// and one unused value, so rows stay 16-byte aligned.
 // WK: incidence rows visited (W, or 4 on meshes of tetrahedra, whose rows 4 and 5 are empty).
 template <typename S, int LIM, int MODE, int WK = W>
 __global__ __launch_bounds__(BLOCK) void k_grad_limit(Topo t, Geom<S> g, const S *__restrict__ P,
@@ -258,7 +261,7 @@ __global__ __launch_bounds__(BLOCK) void k_grad_limit(Topo t, Geom<S> g, const S
     load_rec(pc, P, c, 5);
     for (int v = 0; v < 5; v++) mn[v] = mx[v] = pc[v];
     S gr[15];
-    if (MODE == 0) {
+    if (MODE != 1) {
         for (int i = 0; i < 15; i++) gr[i] = 0;
     }
     else {
@@ -285,7 +288,7 @@ __global__ __launch_bounds__(BLOCK) void k_grad_limit(Topo t, Geom<S> g, const S
         for (int v = 0; v < 5; v++) {
             mn[v] = tmin(mn[v], pn[k][v]);
             mx[v] = tmax(mx[v], pn[k][v]);
-            if (MODE == 0) {
+            if (MODE != 1) {
                 S diff = side ? (pc[v] - pn[k][v]) : (pn[k][v] - pc[v]);  // P[right] - P[left]
                 S s = side ? -diff : diff;
                 gr[(v * 3) + 0] += s * wk[k][0];
@@ -325,6 +328,12 @@ __global__ __launch_bounds__(BLOCK) void k_grad_limit(Topo t, Geom<S> g, const S
                 }
             }
         }
+    }
+    if (MODE == 2) {
+        S *out = &GP[(long)c * 16];
+        for (int i = 0; i < 15; i++) out[i] = gr[i] * ps[i / 3];
+        out[15] = 0;
+        return;
     }
     S *out = &GP[(long)c * 20];
     if (MODE == 0) {
@@ -375,11 +384,21 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
     S pl[5], pr[5];
     load_rec(pl, P, l, 5);
     load_rec(pr, P, r, 5);
+    // gradient records: viscous passes read [G (15), psi (5)] rows; inviscid passes read the
+    // limited gradient G * psi alone, from rows of 16 (k_grad_limit MODE 2)
     S gl[20], gr[20];
     if (ORDER == 2) {
-        load_rec(gl, GP, l, 20);
-        if (!bnd) {
-            load_rec(gr, GP, r, 20);
+        if (VISC) {
+            load_rec(gl, GP, l, 20);
+            if (!bnd) {
+                load_rec(gr, GP, r, 20);
+            }
+        }
+        else {
+            for (int i = 0; i < 15; i++) gl[i] = GP[((long)l * 16) + i];
+            if (!bnd) {
+                for (int i = 0; i < 15; i++) gr[i] = GP[((long)r * 16) + i];
+            }
         }
     }
     const S area = ld_stream(&g.area[f]);
@@ -414,20 +433,19 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
     if (FLUX != FLUX_NONE) {
         S ql[5], qr[5];
         if (ORDER == 2) {  // reconstruct with the limited gradient G*psi (derivative.c:158-164)
-            for (int v = 0; v < 5; v++) {
-                S ps = gl[15 + v];
-                V3<S> gv = v3(gl[(v * 3) + 0] * ps, gl[(v * 3) + 1] * ps, gl[(v * 3) + 2] * ps);
-                ql[v] = pl[v] + dot(gv, ol);
-            }
+            auto limited = [&](const S *gx, int v) {
+                if (VISC) {
+                    S ps = gx[15 + v];
+                    return v3(gx[(v * 3) + 0] * ps, gx[(v * 3) + 1] * ps, gx[(v * 3) + 2] * ps);
+                }
+                return v3(gx[(v * 3) + 0], gx[(v * 3) + 1], gx[(v * 3) + 2]);
+            };
+            for (int v = 0; v < 5; v++) ql[v] = pl[v] + dot(limited(gl, v), ol);
             if (bnd) {  // raw ghost state (derivative.c:203-211)
                 for (int v = 0; v < 5; v++) qr[v] = pr[v];
             }
             else {
-                for (int v = 0; v < 5; v++) {
-                    S ps = gr[15 + v];
-                    V3<S> gv = v3(gr[(v * 3) + 0] * ps, gr[(v * 3) + 1] * ps, gr[(v * 3) + 2] * ps);
-                    qr[v] = pr[v] + dot(gv, orr);
-                }
+                for (int v = 0; v < 5; v++) qr[v] = pr[v] + dot(limited(gr, v), orr);
             }
         }
         else {

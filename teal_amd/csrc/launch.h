@@ -158,6 +158,13 @@ static void grad_limit_wk(hipStream_t s, int lim, int mode, fv::Topo t, fv::Geom
             default: hipLaunchKernelGGL((fv::k_grad_limit<S, 2, 0, WK>), grid, block, 0, s, t, g, P, GP); return;
         }
     }
+    if (mode == 2) {
+        switch (lim) {
+            case 0: hipLaunchKernelGGL((fv::k_grad_limit<S, 0, 2, WK>), grid, block, 0, s, t, g, P, GP); return;
+            case 1: hipLaunchKernelGGL((fv::k_grad_limit<S, 1, 2, WK>), grid, block, 0, s, t, g, P, GP); return;
+            default: hipLaunchKernelGGL((fv::k_grad_limit<S, 2, 2, WK>), grid, block, 0, s, t, g, P, GP); return;
+        }
+    }
     switch (lim) {
         case 1: hipLaunchKernelGGL((fv::k_grad_limit<S, 1, 1, WK>), grid, block, 0, s, t, g, P, GP); return;
         default: hipLaunchKernelGGL((fv::k_grad_limit<S, 2, 1, WK>), grid, block, 0, s, t, g, P, GP); return;
