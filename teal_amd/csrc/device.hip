@@ -1568,19 +1568,25 @@ static void batch_rows(const TealGpu *g, int i, double *uni, double *inst, int t
     }
 }
 
-static int check_batch(TealGpu *g, const void *arrays, const char *who)
+static int check_batch(TealGpu *g, const void *const *arrays, const char *who)
 {
     if (check_ready(g)) return -1;
     if (g->inst_rows.empty() || !arrays) {
         teal_gpu_set_error("%s: not a batch handle, or null arrays", who);
         return -1;
     }
+    for (int i = 0; i < g->ninst; i++) {
+        if (!arrays[i]) {
+            teal_gpu_set_error("%s: null array for instance %d", who, i);
+            return -1;
+        }
+    }
     return 0;
 }
 
 extern "C" int teal_gpu_upload_batch(TealGpu *g, const double *const *primitive)
 {
-    if (check_batch(g, primitive, "upload_batch")) return -1;
+    if (check_batch(g, (const void *const *)primitive, "upload_batch")) return -1;
     std::vector<double> uni((size_t)5 * g->mesh->cells.num, 0.0);
     for (int i = 0; i < g->ninst; i++) {
         batch_rows(g, i, uni.data(), const_cast<double *>(primitive[i]), 1);
@@ -1590,7 +1596,7 @@ extern "C" int teal_gpu_upload_batch(TealGpu *g, const double *const *primitive)
 
 extern "C" int teal_gpu_download_batch(TealGpu *g, double *const *primitive)
 {
-    if (check_batch(g, primitive, "download_batch")) return -1;
+    if (check_batch(g, (const void *const *)primitive, "download_batch")) return -1;
     std::vector<double> uni((size_t)5 * g->mesh->cells.num, 0.0);
     if (teal_gpu_download(g, uni.data())) return -1;
     for (int i = 0; i < g->ninst; i++) {
