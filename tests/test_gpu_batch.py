@@ -53,10 +53,10 @@ def build(sizes, precision=teal_amd.FP64, seed=11):
     return lib, meshes, states, oracles, batch
 
 
-def oracle_run(om, oe, nc, P, steps, courant=0.5):
+def oracle_run(om, oe, nc, P, steps, courant=0.5, t0=0.0):
     po = np.ctypeslib.as_array(oracle().oracle_eqns_primitive(oe), shape=(nc * 5,))
     po[:] = P.ravel()
-    t = C.c_double(0.0)
+    t = C.c_double(t0)
     res = (C.c_double * 5)()
     for _ in range(steps):
         oracle().oracle_lserk(oe, C.byref(t), res, 1e300, courant, 2, 3)
@@ -77,14 +77,16 @@ def test_batch_layout():
 
 
 def test_batch_fp64_matches_each_instance_alone():
+    """Each instance starts at its own time; times advance by the instance's own steps."""
     steps = 12
     print(f"batch fp64 {len(SIZES)} tubes {steps} steps", flush=True)
     _, _, states, oracles, batch = build(SIZES)
     batch.upload_batch(states)
-    times, res = batch.run_lserk_batch(steps, np.zeros(len(SIZES)), 0.5)
+    t0 = 0.25 * np.arange(len(SIZES))
+    times, res = batch.run_lserk_batch(steps, t0, 0.5)
     got = batch.download_batch()
     for i, ((om, oe, nc, ni), P) in enumerate(zip(oracles, states)):
-        Po, t_o, r_o = oracle_run(om, oe, nc, P, steps)
+        Po, t_o, r_o = oracle_run(om, oe, nc, P, steps, t0=t0[i])
         assert times[i] == t_o, (i, times[i], t_o)
         np.testing.assert_allclose(res[i], r_o, rtol=1e-12, atol=1e-300)
         np.testing.assert_array_equal(got[i][:ni], Po[:ni])
@@ -128,6 +130,24 @@ def test_batch_of_tets_and_hexes():
         np.testing.assert_allclose(res[i], r_o, rtol=1e-12, atol=1e-300)
         np.testing.assert_array_equal(got[i][:ni], Po[:ni])
     batch.close()
+
+
+def test_batch_of_one_equals_single_handle():
+    """A batch of one mesh has no padding and steps exactly like a single-mesh handle."""
+    print("batch of one", flush=True)
+    _, meshes, states, _, batch = build(SIZES[4:5])
+    single = teal_amd.Gpu(meshes[0])
+    single.configure(space_order=2, flux="hllc", limiter="minmod",
+                     bcs={ent: (name, ctx) for ent, name, ctx in BCS})
+    assert batch.info()["num_inner"] == single.info()["num_inner"]
+    batch.upload_batch(states)
+    single.upload(states[0])
+    times, _ = batch.run_lserk_batch(7, np.zeros(1), 0.5)
+    t = single.run_lserk(7, 0.0, 0.5)
+    assert times[0] == t
+    np.testing.assert_array_equal(batch.download_batch()[0], single.download())
+    batch.close()
+    single.close()
 
 
 def test_batch_derivative_and_residuals():
