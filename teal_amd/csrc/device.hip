@@ -361,7 +361,7 @@ int DevData<S>::build(const Mesh *m, TealGpu *gpu)
     // incidences: walking faces in reference order keeps each list sorted by reference index
     std::vector<int> cnt(ni, 0);
     std::vector<int> inc((size_t)W * ni, -1), nbr((size_t)W * ni, -1);
-    std::vector<double> w((size_t)3 * W * ni, 0.0), o((size_t)3 * W * ni, 0.0);
+    std::vector<double> w((size_t)6 * W * ni, 0.0);  // LSQ weights and cell offsets, paired
     for (int f = 0; f < nf; f++) {
         int l = m->faces.cell_idx[f].left;
         int r = m->faces.cell_idx[f].right;
@@ -381,11 +381,12 @@ int DevData<S>::build(const Mesh *m, TealGpu *gpu)
             nbr[idx] = ((side ? cinv[l] : resolve(r)) << 1) | side;
             Vector wv = side ? m->faces.weight[f].right : m->faces.weight[f].left;
             Vector ov = side ? m->faces.offset[f].right : m->faces.offset[f].left;
-            double wa[3] = {wv.x, wv.y, wv.z};
-            double oa[3] = {ov.x, ov.y, ov.z};
-            for (int d = 0; d < 3; d++) {
-                w[((((size_t)d * W) + k) * ni) + c] = wa[d];
-                o[((((size_t)d * W) + k) * ni) + c] = oa[d];
+            // pairs (wx, wy), (wz, ox), (oy, oz), each [W][ni][2]: one 2-wide load per pair
+            const double six[6] = {wv.x, wv.y, wv.z, ov.x, ov.y, ov.z};
+            for (int q = 0; q < 3; q++) {
+                size_t at = ((((size_t)q * W) + k) * ni + c) * 2;
+                w[at] = six[2 * q];
+                w[at + 1] = six[(2 * q) + 1];
             }
         }
     }
@@ -436,7 +437,7 @@ int DevData<S>::build(const Mesh *m, TealGpu *gpu)
         corr[(3 * (size_t)nf) + d] = m->faces.correction[f].norm;
     }
     if (upload(&d_inc, inc) || upload(&d_nbr, nbr) || upload(&d_flr, flr) || upload_conv(&d_w, w) ||
-        upload_conv(&d_o, o) || upload_conv(&d_vol, vol) || upload_conv(&d_proj, proj) ||
+        upload_conv(&d_vol, vol) || upload_conv(&d_proj, proj) ||
         upload_conv(&d_area, area) || upload_conv(&d_basis, basis) ||
         upload_conv(&d_offl, offl) || upload_conv(&d_offr, offr) || upload_conv(&d_corr, corr) ||
         upload(&d_vol64, vol)) {

@@ -76,8 +76,9 @@ struct Geom {
     const S *__restrict__ vol;    // [ni]
     const S *__restrict__ proj;   // [3][ni]
     const S *__restrict__ eps2;   // [ni] (VK) or null
-    const S *__restrict__ w;      // [3][W][ni]
-    const S *__restrict__ o;      // [3][W][ni]
+    const S *__restrict__ w;      // [3][W][ni][2]: per incidence the LSQ weight and the
+                                  // cell offset as pairs (wx, wy), (wz, ox), (oy, oz)
+    const S *__restrict__ o;      // unused (the offsets are paired into w)
     const S *__restrict__ area;   // [nf]
     const S *__restrict__ basis;  // [9][nf], or [3][nf] (the normal alone) when `tangents`
     const S *__restrict__ offl;   // [3][nf]
@@ -272,12 +273,24 @@ __global__ __launch_bounds__(BLOCK) void k_grad_limit(Topo t, Geom<S> g, const S
     int nb[WK];
 #pragma unroll
     for (int k = 0; k < WK; k++) nb[k] = ld_stream(&t.nbr[(k * ni) + c]);
-    S pn[WK][5], wk[WK][3];
+    typedef S S2 __attribute__((ext_vector_type(2)));
+    const S2 *wo = reinterpret_cast<const S2 *>(g.w);
+    S pn[WK][5], wk[WK][3], ok[WK][3];
 #pragma unroll
     for (int k = 0; k < WK; k++) {
         const long n = (nb[k] < 0) ? c : (nb[k] >> 1);
         load_rec(pn[k], P, n, 5);
-        for (int d = 0; d < 3; d++) wk[k][d] = ld_stream(&g.w[(((d * W) + k) * ni) + c]);
+        const S2 a = ld_stream(&wo[((0 * W) + k) * ni + c]);
+        const S2 b = ld_stream(&wo[((1 * W) + k) * ni + c]);
+        wk[k][0] = a.x;
+        wk[k][1] = a.y;
+        wk[k][2] = b.x;
+        ok[k][0] = b.y;
+        if (LIM != 0) {
+            const S2 e = ld_stream(&wo[((2 * W) + k) * ni + c]);
+            ok[k][1] = e.x;
+            ok[k][2] = e.y;
+        }
     }
 #pragma unroll
     for (int k = 0; k < WK; k++) {
@@ -301,11 +314,6 @@ __global__ __launch_bounds__(BLOCK) void k_grad_limit(Topo t, Geom<S> g, const S
     for (int v = 0; v < 5; v++) ps[v] = 1;
     if (LIM != 0) {
         S eps2 = (LIM == 2) ? g.eps2[c] : S(0);
-        S ok[WK][3];
-#pragma unroll
-        for (int k = 0; k < WK; k++) {
-            for (int d = 0; d < 3; d++) ok[k][d] = ld_stream(&g.o[(((d * W) + k) * ni) + c]);
-        }
 #pragma unroll
         for (int k = 0; k < WK; k++) {
             if (nb[k] < 0) {
