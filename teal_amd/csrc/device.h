@@ -33,6 +33,7 @@ struct DevData {
     // per-block face lists of the fused kernel (fv::BlockTopo)
     int *d_boff = nullptr, *d_bslot = nullptr;
     int2 *d_bflr = nullptr;
+    S *d_bpairs = nullptr;  // FP32: face records in block-list order as pairs ([5][nj][2])
     S *d_barea = nullptr, *d_bbasis = nullptr, *d_boffl = nullptr, *d_boffr = nullptr,
       *d_bcorr = nullptr;  // face geometry in block-list order (stride num_block_faces)
     S *d_bfcen = nullptr;  // centre form: face centres in block-list order

@@ -588,8 +588,30 @@ int DevData<S>::build_blocks(const std::vector<int> &inc, int ni, const std::vec
             for (int k = 0; k < 4; k++) lcorr[((size_t)k * nj) + j] = corr[((size_t)k * nf) + f];
         }
     }
-    if (upload(&d_boff, off) || upload(&d_bslot, slot) || upload(&d_bflr, lflr) ||
-        upload_conv(&d_barea, larea) || upload_conv(&d_bbasis, lbasis)) {
+    if (upload(&d_boff, off) || upload(&d_bslot, slot) || upload(&d_bflr, lflr)) {
+        return -1;
+    }
+    if (!fp64) {  // (area, nx), (ny, nz), (olx, oly), (olz, orx), (ory, orz) per face
+        std::vector<double> pairs(10 * cap, 0.0);
+        for (int j = 0; j < nj; j++) {
+            const double ten[10] = {larea[j],
+                                    lbasis[j],
+                                    lbasis[(size_t)nj + j],
+                                    lbasis[(2 * (size_t)nj) + j],
+                                    loffl[j],
+                                    loffl[(size_t)nj + j],
+                                    loffl[(2 * (size_t)nj) + j],
+                                    loffr[j],
+                                    loffr[(size_t)nj + j],
+                                    loffr[(2 * (size_t)nj) + j]};
+            for (int q = 0; q < 5; q++) {
+                pairs[((((size_t)q * cap) + j) * 2)] = ten[2 * q];
+                pairs[((((size_t)q * cap) + j) * 2) + 1] = ten[(2 * q) + 1];
+            }
+        }
+        return upload_conv(&d_bpairs, pairs);
+    }
+    if (upload_conv(&d_barea, larea) || upload_conv(&d_bbasis, lbasis)) {
         return -1;
     }
     if (centre_form) {
@@ -615,7 +637,7 @@ Geom<S> DevData<S>::block_geom() const
 {
     return Geom<S>{d_vol,    d_proj,   d_eps2,   d_w,      d_o,    d_barea,
                    d_bbasis, d_boffl,  d_boffr,  d_bcorr,  d_bfcen, d_ccen, d_bcrow,
-                   sizeof(S) == 4};
+                   sizeof(S) == 4, d_bpairs};
 }
 
 template <typename S>
@@ -624,7 +646,7 @@ void DevData<S>::release()
     void *ptrs[] = {d_inc, d_nbr, d_flr, d_w, d_o, d_vol, d_proj, d_area, d_basis, d_offl,
                     d_offr, d_corr, d_vol64, d_bleft, d_bright, d_bface, d_bkind, d_bctx,
                     d_P, d_P2, d_GP, d_F, d_eps2, d_boff, d_bslot, d_bflr, d_barea, d_bbasis,
-                    d_boffl, d_boffr, d_bcorr, d_bfcen, d_bcrow, d_ccen};
+                    d_boffl, d_boffr, d_bcorr, d_bfcen, d_bcrow, d_ccen, d_bpairs};
     for (void *p : ptrs) {
         if (p) {
             (void)hipFree(p);

@@ -93,6 +93,9 @@ struct Geom {
     const int *__restrict__ crow; // [nf] centre row of the right cell (the ghost row of a
                                   // periodic face, whose flr.y is its source cell)
     int tangents = 0;             // 1: basis holds the normal; tangents are rebuilt from it
+    // FP32 face records as 2-wide pairs [5][nf][2]: (area, nx), (ny, nz), (olx, oly),
+    // (olz, orx), (ory, orz); when set, area, basis, offl and offr are not read
+    const S *__restrict__ pairs = nullptr;
 };
 
 // Face tangents from the unit normal n with the construction of src/mesh/generate.c:420-435
@@ -363,9 +366,28 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
     const bool bnd = lr.y < 0;
     const int r = bnd ? ~lr.y : lr.y;
     // face -> cell offsets and the centre correction (generate.c:754-777)
-    V3<S> ol, orr, cu;
-    S cn;
-    if (g.offl) {
+    V3<S> ol, orr, cu, nrm;
+    S cn, area;
+    bool paired = false;
+    if constexpr (sizeof(S) == 4) {
+        if (g.pairs) {
+            typedef float f2 __attribute__((ext_vector_type(2)));
+            const f2 *q = reinterpret_cast<const f2 *>(g.pairs);
+            const f2 a = ld_stream(&q[f]), b = ld_stream(&q[nf + f]), c = ld_stream(&q[(2 * nf) + f]),
+                     d = ld_stream(&q[(3 * nf) + f]), e = ld_stream(&q[(4 * nf) + f]);
+            area = a.x;
+            nrm = v3(a.y, b.x, b.y);
+            ol = v3(c.x, c.y, d.x);
+            orr = v3(d.y, e.x, e.y);
+            V3<S> dl = v3(ol.x - orr.x, ol.y - orr.y, ol.z - orr.z);
+            cn = sqrt(dot(dl, dl));
+            cu = v3(dl.x / cn, dl.y / cn, dl.z / cn);
+            paired = true;
+        }
+    }
+    if (paired) {
+    }
+    else if (g.offl) {
         ol = v3(ld_stream(&g.offl[(0 * nf) + f]), ld_stream(&g.offl[(1 * nf) + f]), ld_stream(&g.offl[(2 * nf) + f]));
         orr = v3(ld_stream(&g.offr[(0 * nf) + f]), ld_stream(&g.offr[(1 * nf) + f]), ld_stream(&g.offr[(2 * nf) + f]));
         if (g.corr) {
@@ -409,7 +431,11 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
             }
         }
     }
-    const S area = ld_stream(&g.area[f]);
+    if (!paired) {
+        area = ld_stream(&g.area[f]);
+        nrm = v3(ld_stream(&g.basis[(0 * nf) + f]), ld_stream(&g.basis[(1 * nf) + f]),
+                 ld_stream(&g.basis[(2 * nf) + f]));
+    }
     if (VISC) {
         const S mu = (S)ph.mu;
         const S prl = (S)ph.prandtl;
@@ -433,7 +459,6 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
             gm[v].y -= a3.y;
             gm[v].z -= a3.z;
         }
-        V3<S> nrm = v3(ld_stream(&g.basis[(0 * nf) + f]), ld_stream(&g.basis[(1 * nf) + f]), ld_stream(&g.basis[(2 * nf) + f]));
         S fl[5];
         viscous_central<S>(fl, pm, gm, gamma, mu, prl, nrm);
         for (int v = 0; v < 5; v++) fv[v] = fl[v] * area;
@@ -463,8 +488,10 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
             }
         }
         S b9[9];
-        if (g.tangents) {
-            for (int i = 0; i < 3; i++) b9[i] = ld_stream(&g.basis[(i * nf) + f]);
+        if (paired || g.tangents) {
+            b9[0] = nrm.x;
+            b9[1] = nrm.y;
+            b9[2] = nrm.z;
             face_tangents(b9);
         }
         else {
