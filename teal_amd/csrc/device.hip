@@ -611,16 +611,25 @@ int DevData<S>::build_blocks(const std::vector<int> &inc, int ni, const std::vec
         }
         return upload_conv(&d_bpairs, pairs);
     }
-    if (upload_conv(&d_barea, larea) || upload_conv(&d_bbasis, lbasis)) {
-        return -1;
-    }
-    if (centre_form) {
-        if (upload(&d_bcrow, lcrow) || upload_conv(&d_bfcen, lfcen) || upload_conv(&d_ccen, ccen)) {
+    if (centre_form) {  // (area, b0), (b1, b2), ..., (b7, b8), (fcx, fcy), (fcz, 0) per face
+        std::vector<double> pairs(14 * cap, 0.0);
+        for (int j = 0; j < nj; j++) {
+            double rec[14] = {larea[j]};
+            for (int k = 0; k < 9; k++) rec[1 + k] = lbasis[((size_t)k * nj) + j];
+            for (int k = 0; k < 3; k++) rec[10 + k] = lfcen[((size_t)k * nj) + j];
+            for (int q = 0; q < 7; q++) {
+                pairs[((((size_t)q * cap) + j) * 2)] = rec[2 * q];
+                pairs[((((size_t)q * cap) + j) * 2) + 1] = rec[(2 * q) + 1];
+            }
+        }
+        if (upload(&d_bcrow, lcrow) || upload_conv(&d_bpairs, pairs) || upload_conv(&d_ccen, ccen)) {
             return -1;
         }
+        return 0;
     }
-    else if (upload_conv(&d_boffl, loffl) || upload_conv(&d_boffr, loffr) ||
-             (fp64 && upload_conv(&d_bcorr, lcorr))) {
+    if (upload_conv(&d_barea, larea) || upload_conv(&d_bbasis, lbasis) ||
+        upload_conv(&d_boffl, loffl) || upload_conv(&d_boffr, loffr) ||
+        upload_conv(&d_bcorr, lcorr)) {
         return -1;
     }
     return 0;

@@ -93,8 +93,9 @@ struct Geom {
     const int *__restrict__ crow; // [nf] centre row of the right cell (the ghost row of a
                                   // periodic face, whose flr.y is its source cell)
     int tangents = 0;             // 1: basis holds the normal; tangents are rebuilt from it
-    // FP32 face records as 2-wide pairs [5][nf][2]: (area, nx), (ny, nz), (olx, oly),
-    // (olz, orx), (ory, orz); when set, area, basis, offl and offr are not read
+    // face records as 2-wide pairs; when set, area, basis, offl, offr and fcen are not read.
+    // FP32 [5][nf][2]: (area, nx), (ny, nz), (olx, oly), (olz, orx), (ory, orz);
+    // FP64 centre form [7][nf][2]: (area, b0), (b1, b2), ..., (b7, b8), (fcx, fcy), (fcz, 0)
     const S *__restrict__ pairs = nullptr;
 };
 
@@ -385,6 +386,32 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
             paired = true;
         }
     }
+    S bp[9];  // FP64 paired records: the full basis
+    if constexpr (sizeof(S) == 8) {
+        if (g.pairs) {
+            typedef double d2 __attribute__((ext_vector_type(2)));
+            const d2 *q = reinterpret_cast<const d2 *>(g.pairs);
+            d2 x[7];
+            for (int k = 0; k < 7; k++) x[k] = ld_stream(&q[(k * nf) + f]);
+            area = x[0].x;
+            bp[0] = x[0].y;
+            for (int k = 1; k < 5; k++) {
+                bp[(2 * k) - 1] = x[k].x;
+                bp[2 * k] = x[k].y;
+            }
+            nrm = v3(bp[0], bp[1], bp[2]);
+            const long rr = g.crow[f];
+            V3<S> fc = v3(x[5].x, x[5].y, x[6].x);
+            V3<S> cl = v3(g.ccen[((long)l * 3) + 0], g.ccen[((long)l * 3) + 1], g.ccen[((long)l * 3) + 2]);
+            V3<S> cr = v3(g.ccen[(rr * 3) + 0], g.ccen[(rr * 3) + 1], g.ccen[(rr * 3) + 2]);
+            ol = v3(fc.x - cl.x, fc.y - cl.y, fc.z - cl.z);
+            orr = v3(fc.x - cr.x, fc.y - cr.y, fc.z - cr.z);
+            V3<S> dl = v3(cr.x - cl.x, cr.y - cl.y, cr.z - cl.z);
+            cn = sqrt(dot(dl, dl));
+            cu = v3(dl.x / cn, dl.y / cn, dl.z / cn);
+            paired = true;
+        }
+    }
     if (paired) {
     }
     else if (g.offl) {
@@ -488,7 +515,10 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
             }
         }
         S b9[9];
-        if (paired || g.tangents) {
+        if (sizeof(S) == 8 && paired) {
+            for (int i = 0; i < 9; i++) b9[i] = bp[i];
+        }
+        else if (paired || g.tangents) {
             b9[0] = nrm.x;
             b9[1] = nrm.y;
             b9[2] = nrm.z;
