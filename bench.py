@@ -515,7 +515,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
-                    help="timed steps (default 1000; 500 for the batch workload)")
+                    help="timed steps (default 1000; 500 for the tet and batch workloads)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--accuracy-steps", type=int, default=None)
@@ -542,8 +542,8 @@ def main():
     if args.cpu_worker:
         cpu_worker(args.cpu_worker)
         return
-    if args.steps is None:
-        args.steps = 500 if args.workload == "batch" else 1000
+    if args.steps is None:  # the step counts of the configurations
+        args.steps = 500 if args.workload in ("batch", "tet") else 1000
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this library drives one GPU per process; runs on several GPUs are "
               "not part of it", file=sys.stderr)
