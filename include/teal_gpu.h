@@ -159,10 +159,12 @@ int teal_gpu_autotune(TealGpu *gpu, int steps, double courant, int time_order, i
 // (0 stops timing); teal_gpu_timing_ms returns the mean duration in ms of kernel 0 (gradient
 // + limiter), 1 (face fluxes; split path only) or 2 (the stage kernel: face fluxes in LDS +
 // accumulation + RK update, or the accumulation alone on the split path) and the launch count.
+// While timing is on, steps run eagerly (no graph replay).
 int teal_gpu_timing(TealGpu *gpu, int capacity);
 double teal_gpu_timing_ms(TealGpu *gpu, int kernel, int *launches);
 
-// Device-layout facts for measurement: inner cells, cells, faces, inner faces, offset of the
+// Device-layout facts for measurement: inner cells (device rows: a batch counts the padding
+// that makes every instance whole 256-row blocks), cells, faces, inner faces, offset of the
 // periodic faces, maximum faces per cell, whether residual passes run the fused stage kernel,
 // and whether it reads the centre form of the face geometry (offsets and corrections
 // recomputed from face and cell centres).
