@@ -187,12 +187,14 @@ def test_batch_union_upload_layout():
     batch.close()
 
 
-def test_batch_graph_replays_match_eager():
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_graph_replays_match_eager(precision):
     """Graph replay (two steps per launch) against eager launches, replay after replay, with
-    the program's own downloads and eager work between replays."""
-    print("batch graph replays", flush=True)
-    _, _, states, _, eager = build(SIZES)
-    _, _, _, _, graph = build(SIZES)
+    the program's own downloads and eager work between replays: bit for bit in both paths."""
+    print(f"batch graph replays {precision}", flush=True)
+    prec = teal_amd.FP64 if precision == "fp64" else teal_amd.MIXED
+    _, _, states, _, eager = build(SIZES, prec)
+    _, _, _, _, graph = build(SIZES, prec)
     graph.set_graphs(True)
     eager.upload_batch(states)
     graph.upload_batch(states)
