@@ -19,7 +19,7 @@ template <typename S>
 struct DevData {
     int *d_inc = nullptr, *d_nbr = nullptr;
     int2 *d_flr = nullptr;
-    S *d_w = nullptr, *d_o = nullptr, *d_vol = nullptr, *d_proj = nullptr;
+    S *d_w = nullptr, *d_vol = nullptr, *d_proj = nullptr;
     S *d_area = nullptr, *d_basis = nullptr, *d_offl = nullptr, *d_offr = nullptr,
       *d_corr = nullptr;
     double *d_vol64 = nullptr;

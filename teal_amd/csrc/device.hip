@@ -644,15 +644,14 @@ BlockTopo DevData<S>::block_topo() const
 template <typename S>
 Geom<S> DevData<S>::block_geom() const
 {
-    return Geom<S>{d_vol,    d_proj,   d_eps2,   d_w,      d_o,    d_barea,
-                   d_bbasis, d_boffl,  d_boffr,  d_bcorr,  d_bfcen, d_ccen, d_bcrow,
-                   sizeof(S) == 4, d_bpairs};
+    return Geom<S>{d_vol,    d_proj,   d_eps2,   d_w,     d_barea,  d_bbasis,
+                   d_boffl,  d_boffr,  d_bcorr,  d_bfcen, d_ccen,   d_bcrow, d_bpairs};
 }
 
 template <typename S>
 void DevData<S>::release()
 {
-    void *ptrs[] = {d_inc, d_nbr, d_flr, d_w, d_o, d_vol, d_proj, d_area, d_basis, d_offl,
+    void *ptrs[] = {d_inc, d_nbr, d_flr, d_w, d_vol, d_proj, d_area, d_basis, d_offl,
                     d_offr, d_corr, d_vol64, d_bleft, d_bright, d_bface, d_bkind, d_bctx,
                     d_P, d_P2, d_GP, d_F, d_eps2, d_boff, d_bslot, d_bflr, d_barea, d_bbasis,
                     d_boffl, d_boffr, d_bcorr, d_bfcen, d_bcrow, d_ccen, d_bpairs};
@@ -666,8 +665,8 @@ void DevData<S>::release()
 template <typename S>
 Geom<S> DevData<S>::geom() const
 {
-    return Geom<S>{d_vol,   d_proj, d_eps2, d_w,     d_o,     d_area, d_basis,
-                   d_offl,  d_offr, d_corr, nullptr, nullptr, nullptr};
+    return Geom<S>{d_vol,  d_proj, d_eps2, d_w,     d_area,  d_basis,
+                   d_offl, d_offr, d_corr, nullptr, nullptr, nullptr};
 }
 
 template struct DevData<double>;

@@ -78,9 +78,8 @@ struct Geom {
     const S *__restrict__ eps2;   // [ni] (VK) or null
     const S *__restrict__ w;      // [3][W][ni][2]: per incidence the LSQ weight and the
                                   // cell offset as pairs (wx, wy), (wz, ox), (oy, oz)
-    const S *__restrict__ o;      // unused (the offsets are paired into w)
     const S *__restrict__ area;   // [nf]
-    const S *__restrict__ basis;  // [9][nf], or [3][nf] (the normal alone) when `tangents`
+    const S *__restrict__ basis;  // [9][nf]
     const S *__restrict__ offl;   // [3][nf]
     const S *__restrict__ offr;   // [3][nf]
     const S *__restrict__ corr;   // [4][nf] unit xyz, norm; null with offl set: derived from
@@ -92,7 +91,6 @@ struct Geom {
     const S *__restrict__ ccen;   // [num_cells][3] cell centres (device rows)
     const int *__restrict__ crow; // [nf] centre row of the right cell (the ghost row of a
                                   // periodic face, whose flr.y is its source cell)
-    int tangents = 0;             // 1: basis holds the normal; tangents are rebuilt from it
     // face records as 2-wide pairs; when set, area, basis, offl, offr and fcen are not read.
     // FP32 [5][nf][2]: (area, nx), (ny, nz), (olx, oly), (olz, orx), (ory, orz);
     // FP64 centre form [7][nf][2]: (area, b0), (b1, b2), ..., (b7, b8), (fcx, fcy), (fcz, 0)
@@ -518,7 +516,7 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
         if (sizeof(S) == 8 && paired) {
             for (int i = 0; i < 9; i++) b9[i] = bp[i];
         }
-        else if (paired || g.tangents) {
+        else if (paired) {  // FP32 records keep the normal alone
             b9[0] = nrm.x;
             b9[1] = nrm.y;
             b9[2] = nrm.z;
