@@ -43,7 +43,7 @@ using namespace fv;
 namespace {
 
 constexpr int RT = 256;        // threads of the vector kernels
-constexpr int MAX_PARTS = 512;  // blocks of a reduction (partial sums read by every block)
+constexpr int MAX_PARTS = 1024;  // blocks of a reduction (partial sums read by every block)
 
 // ---------------------------------------------------------------- compensated sums
 __device__ inline double2 two_sum(double a, double b)
