@@ -137,6 +137,7 @@ struct TealGpu {
     int dt_valid = 0;  // the scalars hold the dt minimum of that state
     int nblocks = 0;
     hipStream_t stream = nullptr;
+    hipEvent_t ev_wait = nullptr;  // host waits on the stream by polling this event
     double *d_stage = nullptr;
     double *d_U0 = nullptr;
     double *d_R = nullptr;
@@ -169,6 +170,12 @@ int teal_residual_pass(TealGpu *g, DevData<S> &d, int mode, double alpha, int la
 template <typename S>
 int teal_timestep_pass(TealGpu *g, DevData<S> &d, int write_u0, double *out);
 void teal_krylov_release(TealGpu *g);
+// wait for the handle's stream by polling an event recorded on it (a blocking
+// synchronization costs tens of microseconds of wake-up per call; the Krylov solver and the
+// single-step entry points synchronize every iteration)
+int teal_stream_wait(TealGpu *g);
+// poll an event until the work before it has completed
+int teal_event_wait(hipEvent_t ev);
 // residual = sqrt(sum V R^2 / sum V) of a device derivative [5][ni] (src/equations/residual.c)
 int teal_residual_norm(TealGpu *g, const double *d_derivative, double *residual);
 void teal_device_numbering(TealGpu *g, const Mesh *m);

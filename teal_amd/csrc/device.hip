@@ -720,6 +720,10 @@ static TealGpu *create_handle(const Mesh *mesh, int precision, int ninst, const 
         teal_gpu_set_error("teal_gpu_create: hipStreamCreate failed");
         rc = -1;
     }
+    if (rc == 0 && hipEventCreateWithFlags(&g->ev_wait, hipEventDisableTiming) != hipSuccess) {
+        teal_gpu_set_error("teal_gpu_create: hipEventCreate failed");
+        rc = -1;
+    }
     // host-layout staging: up to 15 values per mesh row, or 5 per device row
     size_t stage = std::max((size_t)15 * mesh->cells.num, (size_t)5 * g->nc);
     if (rc == 0 &&
@@ -851,6 +855,9 @@ extern "C" void teal_gpu_destroy(TealGpu *g)
     g->ev_grad.release();
     g->ev_face.release();
     teal_krylov_release(g);
+    if (g->ev_wait) {
+        (void)hipEventDestroy(g->ev_wait);
+    }
     if (g->stream) {
         (void)hipStreamDestroy(g->stream);
     }
@@ -947,6 +954,7 @@ extern "C" int teal_gpu_configure(TealGpu *g, const TealGpuConfig *cfg)
         (void)hipGraphExecDestroy(g->graph);
         g->graph = nullptr;
     }
+
     if (g->precision == TEAL_FP64) {
         return configure_impl(g, g->d64, cfg);
     }
@@ -1256,6 +1264,21 @@ extern "C" double teal_gpu_timestep(TealGpu *g, const double *primitive)
     return rc ? NAN : dt;
 }
 
+int teal_event_wait(hipEvent_t ev)
+{
+    hipError_t e;
+    while ((e = hipEventQuery(ev)) == hipErrorNotReady) {
+    }
+    HIPCALL(e);
+    return 0;
+}
+
+int teal_stream_wait(TealGpu *g)
+{
+    HIPCALL(hipEventRecord(g->ev_wait, g->stream));
+    return teal_event_wait(g->ev_wait);
+}
+
 // per-instance residual norms of the partials in d_partials: residual[5 * i + v]
 static int read_residuals(TealGpu *g, double *residual)
 {
@@ -1265,7 +1288,7 @@ static int read_residuals(TealGpu *g, double *residual)
     HIPCALL(hipGetLastError());
     HIPCALL(hipMemcpyAsync(g->h_scalars, g->d_scalars, sizeof(Scalars) * (size_t)g->ninst,
                            hipMemcpyDeviceToHost, g->stream));
-    HIPCALL(hipStreamSynchronize(g->stream));
+    if (teal_stream_wait(g)) return -1;
     for (int i = 0; i < g->ninst; i++) {
         memcpy(&residual[(size_t)i * 5], g->h_scalars[i].residual, sizeof(double) * 5);
     }
@@ -1397,7 +1420,7 @@ static int read_scalars(TealGpu *g)
 {
     HIPCALL(hipMemcpyAsync(g->h_scalars, g->d_scalars, sizeof(Scalars) * (size_t)g->ninst,
                            hipMemcpyDeviceToHost, g->stream));
-    HIPCALL(hipStreamSynchronize(g->stream));
+    if (teal_stream_wait(g)) return -1;
     if (g->h_scalars->error == 1) {
         teal_gpu_set_error("vacuum state detected in riemann solver");
         return -1;

@@ -121,6 +121,22 @@ __global__ __launch_bounds__(RT) void k_reduce(const double2 *__restrict__ part,
     }
 }
 
+// out[idx] = sqrt(total), the last entry of a Hessenberg column, and host[0..idx] = out[0..idx]:
+// the column written to host-visible memory (k_reduce with take_sqrt and the column's copy)
+__global__ __launch_bounds__(RT) void k_reduce_column(const double2 *__restrict__ part, int nb,
+                                                      double *out, int idx, double *host)
+{
+    double t = reduce_parts(part, nb);
+    if (threadIdx.x == 0) {
+        double r = sqrt(t);
+        out[idx] = r;
+        host[idx] = r;
+    }
+    for (int i = threadIdx.x; i < idx; i += RT) {
+        host[i] = out[i];
+    }
+}
+
 // One modified Gram-Schmidt step (gmres.c:92-99): h = the dot whose partials are in part_in
 // (hess[i-1][iter]); w -= h * v_prev; partials of dot(v_next, w) (v_next == w: the norm).
 __global__ __launch_bounds__(RT) void k_mgs(double *__restrict__ w, const double *__restrict__ v_prev,
@@ -258,6 +274,28 @@ __global__ __launch_bounds__(RT) void k_matvec_finish(const double *__restrict__
     }
 }
 
+// k_matvec_finish, then part[block] = partial dot(v0, w) as k_dot(v0, w) forms it: the matvec's
+// second half and the first Gram-Schmidt dot in one pass (the same values and partials)
+__global__ __launch_bounds__(RT) void k_matvec_dot(const double *__restrict__ v,
+                                                   const double *__restrict__ D1,
+                                                   const double *__restrict__ D, double step,
+                                                   const double *eps_ptr, double *__restrict__ w,
+                                                   const double *__restrict__ v0, long n,
+                                                   double2 *__restrict__ part)
+{
+    const double eps = *eps_ptr;
+    double2 acc = make_double2(0.0, 0.0);
+    for (long e = (blockIdx.x * (long)RT) + threadIdx.x; e < n; e += (long)gridDim.x * RT) {
+        double x = v[e] - ((step * (D1[e] - D[e])) / eps);
+        w[e] = x;
+        add_product(acc, v0[e], x);
+    }
+    double2 tot = block_reduce(acc);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = tot;
+    }
+}
+
 // basis[0] = -r / norm (gmres.c:73-77), basis[iter+1] = w / h (gmres.c:121-125):
 // out = (sign * x) / d
 __global__ __launch_bounds__(RT) void k_scale(double *__restrict__ out, const double *__restrict__ x,
@@ -265,6 +303,28 @@ __global__ __launch_bounds__(RT) void k_scale(double *__restrict__ out, const do
 {
     for (long e = (blockIdx.x * (long)RT) + threadIdx.x; e < n; e += (long)gridDim.x * RT) {
         out[e] = (sign * x[e]) / d;
+    }
+}
+
+// basis[iter+1] = w / h with h read on the device (the same expression as k_scale's with sign
+// 1: the next basis vector is formed before the host has read h), and part[block] = partial
+// dot(basis[iter+1], basis[iter+1]) as k_dot forms it (the matvec's eps)
+__global__ __launch_bounds__(RT) void k_scale_dot(double *__restrict__ out, const double *__restrict__ x,
+                                                  const double *__restrict__ d, long n,
+                                                  double2 *__restrict__ part)
+{
+    const double h = *d;
+    double2 acc = make_double2(0.0, 0.0);
+    for (long e = (blockIdx.x * (long)RT) + threadIdx.x; e < n; e += (long)gridDim.x * RT) {
+        double y = x[e] / h;
+        out[e] = y;
+        add_product(acc, y, y);
+    }
+    if (part) {  // uniform across the block
+        double2 tot = block_reduce(acc);
+        if (threadIdx.x == 0) {
+            part[blockIdx.x] = tot;
+        }
     }
 }
 
@@ -338,10 +398,12 @@ struct Krylov {
     double *coef = nullptr;   // back-substitution coefficients (dim)
     double2 *part[2] = {nullptr, nullptr};
     double *scal = nullptr;   // [0..dim]: Hessenberg column, [dim+1]: eps, [dim+2..]: norms
-    double *h_scal = nullptr; // pinned copy of scal
+    double *h_scal = nullptr; // pinned copy of scal (host-coherent: kernels write columns into it)
+    double *h_scal_dev = nullptr;  // h_scal's device address
     int *dev_of_ref = nullptr; // serial reductions: device row of each inner cell (reference order)
     long newton = 0, krylov = 0;  // iterations of the last implicit step (Newton, GMRES)
 };
+
 
 void teal_krylov_release(TealGpu *g)
 {
@@ -386,7 +448,8 @@ static int krylov_reserve(TealGpu *g, int dim)
     HIPCALL(hipMalloc((void **)&k->part[0], sizeof(double2) * MAX_PARTS));
     HIPCALL(hipMalloc((void **)&k->part[1], sizeof(double2) * MAX_PARTS));
     HIPCALL(hipMalloc((void **)&k->scal, sizeof(double) * (size_t)(dim + 8)));
-    HIPCALL(hipHostMalloc((void **)&k->h_scal, sizeof(double) * (size_t)(dim + 8), hipHostMallocDefault));
+    HIPCALL(hipHostMalloc((void **)&k->h_scal, sizeof(double) * (size_t)(dim + 8), hipHostMallocCoherent));
+    HIPCALL(hipHostGetDevicePointer((void **)&k->h_scal_dev, k->h_scal, 0));
     HIPCALL(hipMalloc((void **)&k->dev_of_ref, sizeof(int) * (size_t)g->ni));
     HIPCALL(hipMemcpy(k->dev_of_ref, g->cell_inv.data(), sizeof(int) * (size_t)g->ni,
                       hipMemcpyHostToDevice));
@@ -417,7 +480,7 @@ int two_norms(TealGpu *g, Krylov *k, const double *a, const double *b, int slot)
     HIPCALL(hipGetLastError());
     HIPCALL(hipMemcpyAsync(k->h_scal + slot, k->scal + slot, 2 * sizeof(double), hipMemcpyDeviceToHost,
                            g->stream));
-    HIPCALL(hipStreamSynchronize(g->stream));
+    if (teal_stream_wait(g)) return -1;
     return 0;
 }
 
@@ -448,35 +511,40 @@ int gmres(TealGpu *g, DevData<S> &d, Krylov *k, const double *Dbase, double step
     std::vector<double> hess((size_t)(dim + 1) * dim, 0.0);
     auto H = [&](int i, int j) -> double & { return hess[((size_t)i * dim) + j]; };
     rhs[0] = norm;
+    double *eps_dev = k->scal + dim + 1;  // eps of the current matvec
     hipLaunchKernelGGL(k_scale, dim3(nb), dim3(RT), 0, g->stream, k->V, (const double *)k->Rn, -1.0,
                        norm, n);
     const double tol_norm = tol * norm;
-    double *eps_dev = k->scal + dim + 1;
-    int iter = 0;
-    for (; iter < dim; iter++) {
-        double *v = k->V + ((size_t)iter * n);
-        // matvec (gmres.c:12-50): w = v - step (R(cons2prim(U + eps v)) - R(U)) / eps
+    // first half of the matvec (gmres.c:12-50) for basis vector j: eps = fd_scale / |v| into
+    // scal[dim + 1], the primitive state of U + eps v into d.d_P and its residual into d_R
+    // (tree reductions: the partials of |v|^2 are in part[0] already when `have_parts`)
+    auto perturbed_residual = [&](int j, bool have_parts) -> int {
+        const double *v = k->V + ((size_t)j * n);
         const double2 *eps_parts = nullptr;
-        if (g->serial_reductions) {  // eps = fd_scale / |v| into scal[dim + 1]
-            hipLaunchKernelGGL(k_serial_dot, dim3(1), dim3(RT), 0, g->stream, (const double *)v,
-                               (const double *)v, (const int *)k->dev_of_ref, g->ni, k->scal, dim + 1,
-                               2, fd_scale);
+        if (g->serial_reductions) {
+            hipLaunchKernelGGL(k_serial_dot, dim3(1), dim3(RT), 0, g->stream, v, v,
+                               (const int *)k->dev_of_ref, g->ni, k->scal, dim + 1, 2, fd_scale);
         }
         else {
-            hipLaunchKernelGGL(k_dot, dim3(nb), dim3(RT), 0, g->stream, (const double *)v,
-                               (const double *)v, n, k->part[0]);
+            if (!have_parts) {
+                hipLaunchKernelGGL(k_dot, dim3(nb), dim3(RT), 0, g->stream, v, v, n, k->part[0]);
+            }
             eps_parts = k->part[0];
         }
         hipLaunchKernelGGL((k_perturb<S>), dim3(cg), dim3(RT), 0, g->stream, (const double *)k->U,
-                           (const double *)v, g->ni, eps_parts, nb, fd_scale, gamma, eps_dev, d.d_P);
+                           v, g->ni, eps_parts, nb, fd_scale, gamma, eps_dev, d.d_P);
         HIPCALL(hipGetLastError());
-        if (teal_residual_pass(g, d, 0, 0.0, 0)) return -1;
-        hipLaunchKernelGGL(k_matvec_finish, dim3(nb), dim3(RT), 0, g->stream, (const double *)v,
-                           (const double *)g->d_R, Dbase, step, (const double *)eps_dev,
-                           k->w, n);
-        // modified Gram-Schmidt (gmres.c:92-100): hess[i][iter] lands in scal[i]
+        return teal_residual_pass(g, d, 0, 0.0, 0);
+    };
+    // second half of the matvec, w = v - step (R(cons2prim(U + eps v)) - R(U)) / eps, then
+    // modified Gram-Schmidt (gmres.c:92-100): hess[i][j] lands in scal[i], and h_scal
+    // receives the column
+    auto column = [&](int j) -> int {
+        const double *vj = k->V + ((size_t)j * n);
         if (g->serial_reductions) {
-            for (int i = 0; i <= iter; i++) {
+            hipLaunchKernelGGL(k_matvec_finish, dim3(nb), dim3(RT), 0, g->stream, vj,
+                               (const double *)g->d_R, Dbase, step, (const double *)eps_dev, k->w, n);
+            for (int i = 0; i <= j; i++) {
                 const double *vi = k->V + ((size_t)i * n);
                 hipLaunchKernelGGL(k_serial_dot, dim3(1), dim3(RT), 0, g->stream, vi,
                                    (const double *)k->w, (const int *)k->dev_of_ref, g->ni, k->scal, i,
@@ -486,26 +554,52 @@ int gmres(TealGpu *g, DevData<S> &d, Krylov *k, const double *Dbase, double step
             }
             hipLaunchKernelGGL(k_serial_dot, dim3(1), dim3(RT), 0, g->stream, (const double *)k->w,
                                (const double *)k->w, (const int *)k->dev_of_ref, g->ni, k->scal,
-                               iter + 1, 1, 0.0);
+                               j + 1, 1, 0.0);
+            HIPCALL(hipGetLastError());
+            HIPCALL(hipMemcpyAsync(k->h_scal, k->scal, sizeof(double) * (size_t)(j + 2),
+                                   hipMemcpyDeviceToHost, g->stream));
         }
         else {
-            hipLaunchKernelGGL(k_dot, dim3(nb), dim3(RT), 0, g->stream, (const double *)k->V,
-                               (const double *)k->w, n, k->part[0]);
+            hipLaunchKernelGGL(k_matvec_dot, dim3(nb), dim3(RT), 0, g->stream, vj,
+                               (const double *)g->d_R, Dbase, step, (const double *)eps_dev, k->w,
+                               (const double *)k->V, n, k->part[0]);
             int cur = 0;
-            for (int i = 0; i <= iter; i++) {
-                const double *next = (i < iter) ? k->V + ((size_t)(i + 1) * n) : nullptr;
+            for (int i = 0; i <= j; i++) {
+                const double *next = (i < j) ? k->V + ((size_t)(i + 1) * n) : nullptr;
                 hipLaunchKernelGGL(k_mgs, dim3(nb), dim3(RT), 0, g->stream, k->w,
                                    (const double *)(k->V + ((size_t)i * n)), next, n,
                                    (const double2 *)k->part[cur], nb, k->part[1 - cur], k->scal, i);
                 cur = 1 - cur;
             }
-            hipLaunchKernelGGL(k_reduce, dim3(1), dim3(RT), 0, g->stream, (const double2 *)k->part[cur],
-                               nb, k->scal, iter + 1, 1);
+            hipLaunchKernelGGL(k_reduce_column, dim3(1), dim3(RT), 0, g->stream,
+                               (const double2 *)k->part[cur], nb, k->scal, j + 1, k->h_scal_dev);
+            HIPCALL(hipGetLastError());
         }
-        HIPCALL(hipGetLastError());
-        HIPCALL(hipMemcpyAsync(k->h_scal, k->scal, sizeof(double) * (size_t)(iter + 2),
-                               hipMemcpyDeviceToHost, g->stream));
-        HIPCALL(hipStreamSynchronize(g->stream));
+        return 0;
+    };
+    // basis vector j + 1 = w / hess[j + 1][j], read on the device, and its perturbed residual
+    auto advance = [&](int j) -> int {
+        double2 *parts = g->serial_reductions ? nullptr : k->part[0];
+        hipLaunchKernelGGL(k_scale_dot, dim3(nb), dim3(RT), 0, g->stream,
+                           k->V + ((size_t)(j + 1) * n), (const double *)k->w,
+                           (const double *)(k->scal + j + 1), n, parts);
+        return perturbed_residual(j + 1, parts != nullptr);
+    };
+    // The next basis vector and its perturbed residual are queued before the host waits for
+    // the Hessenberg column, so the device works while the host reads it and applies the
+    // rotations.  When the iteration converges that work is discarded: it writes only the next
+    // basis vector, eps, the perturbed state and d_R, which the caller overwrites (the Newton
+    // update rewrites d.d_P, the next residual d_R).  The exact Riemann solver is left out: its
+    // vacuum flag is sticky, and an evaluation the reference does not make must not raise it.
+    const bool ahead = g->cfg.flux != TEAL_FLUX_GODUNOV;
+    if (perturbed_residual(0, false)) return -1;
+    int iter = 0;
+    for (; iter < dim; iter++) {
+        const bool next = iter + 1 < dim;
+        if (column(iter)) return -1;
+        HIPCALL(hipEventRecord(g->ev_wait, g->stream));
+        if (next && ahead && advance(iter)) return -1;
+        if (teal_event_wait(g->ev_wait)) return -1;
         for (int i = 0; i <= iter + 1; i++) {
             H(i, iter) = k->h_scal[i];
         }
@@ -526,8 +620,12 @@ int gmres(TealGpu *g, DevData<S> &d, Krylov *k, const double *Dbase, double step
             iter += 1;
             break;
         }
-        hipLaunchKernelGGL(k_scale, dim3(nb), dim3(RT), 0, g->stream, k->V + ((size_t)(iter + 1) * n),
-                           (const double *)k->w, 1.0, H(iter + 1, iter), n);
+        if (next && !ahead) {
+            hipLaunchKernelGGL(k_scale, dim3(nb), dim3(RT), 0, g->stream,
+                               k->V + ((size_t)(iter + 1) * n), (const double *)k->w, 1.0,
+                               H(iter + 1, iter), n);
+            if (perturbed_residual(iter + 1, false)) return -1;
+        }
     }
     // back substitution (gmres.c:127-140)
     for (int i = iter - 1; i >= 0; i--) {
@@ -616,7 +714,7 @@ int implicit_step(TealGpu *g, DevData<S> &dm, double *time, double *residual, do
                            vol64, (const int *)k->dev_of_ref, g->ni, g->sum_volume, k->scal);
         HIPCALL(hipGetLastError());
         HIPCALL(hipMemcpyAsync(k->h_scal, k->scal, 5 * sizeof(double), hipMemcpyDeviceToHost, g->stream));
-        HIPCALL(hipStreamSynchronize(g->stream));
+        if (teal_stream_wait(g)) return -1;
         memcpy(residual, k->h_scal, 5 * sizeof(double));
     }
     else if (teal_residual_norm(g, k->D, residual)) {
