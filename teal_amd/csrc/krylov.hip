@@ -144,12 +144,36 @@ __global__ __launch_bounds__(RT) void k_mgs(double *__restrict__ w, const double
                                             const double2 *__restrict__ part_in, int nb,
                                             double2 *__restrict__ part_out, double *h, int hidx)
 {
+    // the operands of a thread's first PF elements are loaded before the reduction of
+    // part_in, so that their latency and the reduction's overlap (same order of operations)
+    constexpr int PF = 8;
+    const long stride = (long)gridDim.x * RT;
+    const long e0 = (blockIdx.x * (long)RT) + threadIdx.x;
+    double wp[PF], vp[PF], np[PF];
+#pragma unroll
+    for (int k = 0; k < PF; k++) {
+        const long e = e0 + (k * stride);
+        if (e < n) {
+            wp[k] = w[e];
+            vp[k] = v_prev[e];
+            np[k] = v_next ? v_next[e] : 0.0;
+        }
+    }
     const double hv = reduce_parts(part_in, nb);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         h[hidx] = hv;
     }
     double2 acc = make_double2(0.0, 0.0);
-    for (long e = (blockIdx.x * (long)RT) + threadIdx.x; e < n; e += (long)gridDim.x * RT) {
+#pragma unroll
+    for (int k = 0; k < PF; k++) {
+        const long e = e0 + (k * stride);
+        if (e < n) {
+            double x = wp[k] - (hv * vp[k]);
+            w[e] = x;
+            add_product(acc, v_next ? np[k] : x, x);
+        }
+    }
+    for (long e = e0 + (PF * stride); e < n; e += stride) {
         double x = w[e] - (hv * v_prev[e]);
         w[e] = x;
         add_product(acc, v_next ? v_next[e] : x, x);
@@ -243,6 +267,15 @@ __global__ __launch_bounds__(RT) void k_perturb(const double *__restrict__ U, co
                                                 double fd_scale, double gamma, double *eps_io,
                                                 S *__restrict__ P)
 {
+    // the first cell's U and v are loaded before the reduction of the partials (overlap)
+    const int c0 = (blockIdx.x * RT) + threadIdx.x;
+    double u0[5], v0[5];
+    if (c0 < ni) {
+        for (int k = 0; k < 5; k++) {
+            u0[k] = U[((long)k * ni) + c0];
+            v0[k] = v[((long)k * ni) + c0];
+        }
+    }
     double eps;
     if (part) {
         eps = fd_scale / sqrt(reduce_parts(part, nb));
@@ -253,7 +286,13 @@ __global__ __launch_bounds__(RT) void k_perturb(const double *__restrict__ U, co
     else {
         eps = *eps_io;
     }
-    for (int c = (blockIdx.x * RT) + threadIdx.x; c < ni; c += gridDim.x * RT) {
+    if (c0 < ni) {
+        double u[5], p[5];
+        for (int k = 0; k < 5; k++) u[k] = u0[k] + (eps * v0[k]);
+        cons2prim<double>(p, u, gamma);
+        for (int k = 0; k < 5; k++) P[((long)c0 * 5) + k] = (S)p[k];
+    }
+    for (int c = c0 + (gridDim.x * RT); c < ni; c += gridDim.x * RT) {
         double u[5], p[5];
         for (int k = 0; k < 5; k++) u[k] = U[((long)k * ni) + c] + (eps * v[((long)k * ni) + c]);
         cons2prim<double>(p, u, gamma);
