@@ -68,20 +68,29 @@ __device__ inline void add_product(double2 &acc, double a, double b)
     acc = make_double2(t.x, acc.y + (t.y + r));
 }
 
-// block-wide reduction of (sum, error) pairs in a fixed tree; result valid in thread 0
+// block-wide reduction of (sum, error) pairs in a fixed order, the result in every thread: a
+// butterfly across the 64 lanes of each wave (combine is commutative bit for bit: TwoSum's
+// error term is exact, so both partners of an exchange hold the same pair), then the waves'
+// pairs in wave order.  One barrier pair instead of one per tree level.
 __device__ inline double2 block_reduce(double2 v)
 {
-    __shared__ double2 red[RT];
-    red[threadIdx.x] = v;
-    __syncthreads();
-    for (int h = RT / 2; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h) {
-            red[threadIdx.x] = combine(red[threadIdx.x], red[threadIdx.x + h]);
-        }
-        __syncthreads();
+    constexpr int WAVES = RT / 64;
+    __shared__ double2 red[WAVES];
+    for (int m = 32; m > 0; m >>= 1) {
+        double2 o;
+        o.x = __shfl_xor(v.x, m, 64);
+        o.y = __shfl_xor(v.y, m, 64);
+        v = combine(v, o);
     }
-    double2 out = red[0];
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6] = v;
+    }
     __syncthreads();
+    double2 out = red[0];
+    for (int w = 1; w < WAVES; w++) {
+        out = combine(out, red[w]);
+    }
+    __syncthreads();  // red is reused by the next call
     return out;
 }
 
