@@ -31,7 +31,7 @@
 //   k_flux_block one workgroup per block of 256 device cells: the viscous and convective flux
 //                of every face touching the block (derivative.c:91-227) into LDS, then per
 //                cell the viscous contributions in face order, the convective ones (FP32:
-//                the merged records, compensated), R = -R/V (derivative.c:229-267) and, in
+//                the merged records, summed in FP64), R = -R/V (derivative.c:229-267) and, in
 //                RK mode, the stage update + cons2prim (src/simulation/advance.c:101-109);
 //                the last stage also produces the next step's U0, the dt minimum and
 //                residual partial sums
@@ -155,7 +155,9 @@ __device__ inline int xcd_block()
 }
 
 // Face-contribution accumulator.  FP64: plain sequential sums (the reference's order);
-// FP32: compensated (TwoSum) accumulation, read out in FP64.
+// FP32: the FP32 face values summed in FP64 (a conversion and one FP64 addition per value,
+// the same registers as an FP32 TwoSum pair and fewer instructions; the sum of a cell's few
+// FP32 values is exact unless their magnitudes span more than 29 binary orders).
 template <typename S>
 struct Acc;
 
@@ -173,27 +175,20 @@ struct Acc<double> {
 
 template <>
 struct Acc<float> {
-    float hi[5], lo[5];
+    double v[5];
     __device__ void zero()
     {
-        for (int i = 0; i < 5; i++) hi[i] = lo[i] = 0;
+        for (int i = 0; i < 5; i++) v[i] = 0;
     }
-    __device__ void add(int i, float x)
-    {
-        float s = hi[i] + x;
-        float bp = s - hi[i];
-        float err = (hi[i] - (s - bp)) + (x - bp);
-        hi[i] = s;
-        lo[i] += err;
-    }
-    __device__ void sub(int i, float x) { add(i, -x); }
-    __device__ double get(int i) const { return (double)hi[i] + (double)lo[i]; }
+    __device__ void add(int i, float x) { v[i] = v[i] + (double)x; }
+    __device__ void sub(int i, float x) { v[i] = v[i] - (double)x; }
+    __device__ double get(int i) const { return v[i]; }
 };
 
 // Values kept per face between the face phase and the accumulation.  FP64 keeps the viscous
 // and convective contributions apart, for the reference's order (every viscous face of a
 // cell, then every convective one: derivative.c:116-153, 191-224).  FP32 merges them into one
-// record (Fc - Fv) A per face, summed into the cell with one compensated addition.
+// record (Fc - Fv) A per face, summed into the cell with one FP64 addition.
 template <typename S, int VISC, int CONV>
 struct FaceRec {
     static constexpr bool MERGE = sizeof(S) == 4 && VISC && CONV;

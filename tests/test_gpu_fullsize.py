@@ -5,11 +5,11 @@ On a periodic mesh every face flux leaves one cell and enters the other, so the 
 integrals of the conserved variables do not change from step to step.  The FP64 path sums
 each cell's face contributions in the reference's order and updates in FP64: the integrals
 move only by the rounding of the per-cell updates.  The mixed path adds each face's FP32
-record to both cells with compensated sums and updates U in FP64: the same holds, to the
-rounding of the compensated sums.  Both are checked over 10 steps, the integrals summed
+record to both cells in FP64 and updates U in FP64: the same holds, to the rounding of
+those sums and updates.  Both are checked over 10 steps, the integrals summed
 exactly (a naive sum of 2 M terms alone drifts by about 4e-11); the bounds are relative to
 the integral of |U| (for momentum, of its magnitude: those integrals start at zero).
-Measured: FP64 0 for mass and below 1e-18 otherwise; mixed 2.7e-12 for mass, 1.3e-13 and
+Measured: FP64 0 for mass and below 1e-18 otherwise; mixed 3.0e-12 for mass, 2.3e-13 and
 below for momentum and energy."""
 import math
 

@@ -1,4 +1,4 @@
-"""Mixed path (FP32 flux / gradient / geometry, compensated FP32 accumulation, FP64 update)
+"""Mixed path (FP32 flux / gradient / geometry, FP64 accumulation and update)
 against the FP64 oracle, on both the fused stage kernel and the face + accumulation kernels.  Its error is stated, not zero: the bounds below are the ones this
 path is held to (relative to the largest magnitude of each quantity)."""
 import ctypes as C
