@@ -28,8 +28,8 @@ Mesh *teal_mesh_union(const Mesh *const *meshes, int num);
 
 enum TealPrecision {
     TEAL_FP64 = 0,  // FP64 storage and arithmetic: bit-compatible with the reference
-    TEAL_MIXED = 1, // FP32 states/gradients/geometry/flux math, compensated FP32 face
-                    // accumulation, FP64 solution update, dt and residual norm
+    TEAL_MIXED = 1, // FP32 states/gradients/geometry/flux math, FP64 face accumulation,
+                    // FP64 solution update, dt and residual norm
 };
 
 // flux ids follow the reference's name table (src/euler/convective.c:593-623)
