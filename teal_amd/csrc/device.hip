@@ -104,8 +104,9 @@ __global__ __launch_bounds__(DT_THREADS) void k_dt_min(Scalars *scalars, const d
 // (coalesced reads) into STEP_END_SUMS running sums taken round-robin (that many loads in
 // flight), adds those pairwise, then a fixed tree over the 128 partial sums.
 constexpr int STEP_END_THREADS = 640;
-constexpr int STEP_END_SUMS = 8;
-static_assert(STEP_END_THREADS % 5 == 0 && STEP_END_SUMS == 8, "k_step_end layout");
+constexpr int STEP_END_SUMS = 16;
+static_assert(STEP_END_THREADS % 5 == 0 && (STEP_END_SUMS & (STEP_END_SUMS - 1)) == 0,
+              "k_step_end layout");
 
 __global__ __launch_bounds__(STEP_END_THREADS) void k_step_end(Scalars *scalars, const double *partials,
                                                                const int *blk_off,
@@ -118,7 +119,7 @@ __global__ __launch_bounds__(STEP_END_THREADS) void k_step_end(Scalars *scalars,
     const long e1 = (long)blk_off[blockIdx.x + 1] * 5;
     const int t = threadIdx.x;
     const int lanes = STEP_END_THREADS / 5;
-    double acc[STEP_END_SUMS] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double acc[STEP_END_SUMS] = {};
     long e = e0 + t;
     for (; e + ((STEP_END_SUMS - 1) * STEP_END_THREADS) < e1; e += STEP_END_SUMS * STEP_END_THREADS) {
         for (int u = 0; u < STEP_END_SUMS; u++) acc[u] += partials[e + (u * STEP_END_THREADS)];
