@@ -111,6 +111,29 @@ void oracle_eqns_set_bc(OEqns *e, const char *entity, const char *name, const do
     o_fatal("invalid entity (%s)", entity);
 }
 
+void oracle_eqns_set_forcing(OEqns *e, double velocity, double relaxation)
+{
+    e->source = O_SOURCE_FORCING;
+    e->forcing[0] = velocity;
+    e->forcing[1] = relaxation;
+}
+
+void oracle_average(const OEqns *e, const char *entity, const double *p, double *avg)
+{
+    o_average(e, entity, (const double (*)[5])p, avg);
+}
+
+void oracle_norm(const OEqns *e, const double *ref, const double *p, double *norm)
+{
+    o_norm(e, (const double (*)[5])ref, (const double (*)[5])p, norm);
+}
+
+void oracle_polar(const OEqns *e, int entity, double *p, const double *ref, double length,
+                  double *coef, double *pc)
+{
+    o_polar(e, entity, (double (*)[5])p, ref, length, coef, pc);
+}
+
 double *oracle_eqns_primitive(OEqns *e) { return &e->primitive[0][0]; }
 double *oracle_eqns_conserved(OEqns *e) { return &e->conserved[0][0]; }
 double *oracle_eqns_eps2(OEqns *e) { return e->eps2; }

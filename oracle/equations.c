@@ -297,7 +297,98 @@ static void convective_o2(const OEqns *e, const double (*p)[5], double (*r)[5], 
     }
 }
 
-/* src/equations/derivative.c:269-305 (no source term: finalize is R = -R/V) */
+/* src/equations/average.c:8-41: volume-weighted sum over the cells of an inner entity,
+ * divided by the volume of the whole mesh (sum_volume), in cell order */
+void o_average(const OEqns *e, const char *entity, const double (*p)[5], double *avg)
+{
+    const OMesh *m = e->mesh;
+    memset(avg, 0, 5 * sizeof(double));
+    for (int i = 0; i < m->entities.num_inner; i++) {
+        if (!strcmp(m->entities.name[i], entity)) {
+            for (int j = m->entities.cell_off[i]; j < m->entities.cell_off[i + 1]; j++) {
+                for (int k = 0; k < 5; k++) {
+                    avg[k] += m->cells.volume[j] * p[j][k];
+                }
+            }
+            for (int k = 0; k < 5; k++) {
+                avg[k] /= m->cells.sum_volume;
+            }
+            return;
+        }
+    }
+    o_fatal("invalid entity (%s)", entity);
+}
+
+/* src/equations/norm.c:8-38 with the reference solution given per inner cell */
+void o_norm(const OEqns *e, const double (*ref)[5], const double (*p)[5], double *norm)
+{
+    const OMesh *m = e->mesh;
+    memset(norm, 0, 5 * sizeof(double));
+    for (int i = 0; i < m->cells.num_inner; i++) {
+        for (int j = 0; j < 5; j++) {
+            double d = ref[i][j] - p[i][j];
+            norm[j] += m->cells.volume[i] * (d * d);
+        }
+    }
+    for (int j = 0; j < 5; j++) {
+        norm[j] = sqrt(norm[j] / m->cells.sum_volume);
+    }
+}
+
+/* src/euler/polar.c:22-83 without the printing and the file: after the boundary pass, the
+ * force coefficients (x, y) of the entity's faces summed in face order and each face's
+ * pressure coefficient; reference = free-stream primitive state */
+void o_polar(const OEqns *e, int entity, double (*p)[5], const double *ref, double length,
+             double *coef, double *pc)
+{
+    const OMesh *m = e->mesh;
+    o_boundary(e, p);
+    double dyn = ref[0] * (((ref[1] * ref[1]) + (ref[2] * ref[2])) + (ref[3] * ref[3])) / 2;
+    coef[0] = coef[1] = 0;
+    for (int i = m->entities.face_off[entity], j = 0; i < m->entities.face_off[entity + 1];
+         i++, j++) {
+        int left = m->faces.cell_idx[i].left;
+        int right = m->faces.cell_idx[i].right;
+        double area = m->faces.area[i];
+        OVec normal = m->faces.basis[i].x;
+        double pressure = (p[left][4] + p[right][4]) / 2;
+        pc[j] = (pressure - ref[4]) / dyn;
+        coef[0] += (pressure * area * normal.x) / (dyn * length);
+        coef[1] += (pressure * area * normal.y) / (dyn * length);
+    }
+}
+
+/* finalize_derivative, src/equations/derivative.c:229-267: R = -R/V, plus the source term of
+ * the cell's primitive state when one is set.  The forcing source is the periodic-hill
+ * driver's (run/periodic_hill/run.c:8-21): its Prepare hook averages the state over the
+ * entity "domain" once per evaluation; per cell s1 = rho (U - <u>) / tau goes to the x
+ * momentum and u s1 to the energy, the other components stay 0. */
+static void finalize(const OEqns *e, const double (*p)[5], double (*r)[5])
+{
+    const OMesh *m = e->mesh;
+    int ni = m->cells.num_inner;
+    if (e->source == O_SOURCE_NONE) {
+        for (int i = 0; i < ni; i++) {
+            for (int j = 0; j < 5; j++) {
+                r[i][j] = -r[i][j] / m->cells.volume[i];
+            }
+        }
+        return;
+    }
+    double avg[5];
+    o_average(e, "domain", p, avg);
+    double src[5] = {0, 0, 0, 0, 0};
+    for (int i = 0; i < ni; i++) {
+        double s1 = p[i][0] * (e->forcing[0] - avg[1]) / e->forcing[1];
+        src[1] = s1;
+        src[4] = p[i][1] * s1;
+        for (int j = 0; j < 5; j++) {
+            r[i][j] = (-r[i][j] / m->cells.volume[i]) + src[j];
+        }
+    }
+}
+
+/* src/equations/derivative.c:269-305 */
 void o_derivative(const OEqns *e, double (*p)[5], double (*r)[5])
 {
     const OMesh *m = e->mesh;
@@ -321,11 +412,7 @@ void o_derivative(const OEqns *e, double (*p)[5], double (*r)[5])
         }
         free(g);
     }
-    for (int i = 0; i < ni; i++) {
-        for (int j = 0; j < 5; j++) {
-            r[i][j] = -r[i][j] / m->cells.volume[i];
-        }
-    }
+    finalize(e, (const double (*)[5])p, r);
 }
 
 /* src/equations/residual.c:8-33 */

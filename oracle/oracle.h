@@ -17,7 +17,7 @@
  *     order/stage pair, implicit_euler) and GMRES: bit-for-bit against the
  *     reference's own sources compiled into oracle/_ref (see oracle/Makefile)
  *     and against the golden fixtures written from that build
- *     (tests/golden/*.npz); plus closed forms (flux consistency, supersonic
+ *     (the .npz files under tests/golden); plus closed forms (flux consistency, supersonic
  *     upwinding, rotation and mirror invariance, Toro's exact-Riemann star
  *     states, BC invariants, viscous stress and Fourier heat flux, LSQ
  *     exactness on linear fields and the reference's convergence-order test,
@@ -200,7 +200,13 @@ typedef struct {
     double (*bc_ctx)[5]; /* context state per boundary entity */
     double (*primitive)[5];
     double (*conserved)[5];
+    /* source term: O_SOURCE_FORCING is the bulk-velocity forcing of
+     * run/periodic_hill/run.c:8-21 (velocity, relaxation time) */
+    int source;
+    double forcing[2];
 } OEqns;
+
+enum { O_SOURCE_NONE = 0, O_SOURCE_FORCING = 1 };
 
 OEqns *oeqns_create(const OMesh *mesh, int navierstokes);
 void oeqns_destroy(OEqns *eqns);
@@ -216,6 +222,12 @@ void o_gradient(const OEqns *eqns, double (*primitive)[5], OVec (*gradient)[5]);
 void o_limiter(const OEqns *eqns, const double (*primitive)[5], OVec (*gradient)[5]);
 void o_derivative(const OEqns *eqns, double (*primitive)[5], double (*derivative)[5]);
 void o_residual(const OEqns *eqns, const double (*derivative)[5], double *residual);
+void o_average(const OEqns *eqns, const char *entity, const double (*primitive)[5],
+               double *average);
+void o_norm(const OEqns *eqns, const double (*reference)[5], const double (*primitive)[5],
+            double *norm);
+void o_polar(const OEqns *eqns, int entity, double (*primitive)[5], const double *reference,
+             double length, double *coef, double *pressure_c);
 
 /* physics plugins (src/euler/convective.c, boundary.c, create.c, riemann.c; src/navierstokes) */
 void o_convective(int kind, double *flux, const double *left, const double *right,

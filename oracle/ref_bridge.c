@@ -101,6 +101,41 @@ void ref_eqns_set_bc(Equations *eqns, const char *entity, const char *name, cons
     equations_set_boundary_condition(eqns, entity, name, 0, context);
 }
 
+/* The periodic-hill driver's source term (run/periodic_hill/run.c:8-21) with its bulk
+ * velocity and relaxation time as parameters: the reference calls the Prepare hook once per
+ * equations_derivative and the source per inner cell (src/equations/derivative.c:229-267). */
+static double forcing_velocity = 1, forcing_relaxation = 0.001;
+static NavierStokesPrimitive forcing_average;
+
+static void forcing_prepare(const Equations *eqns, const void *primitive)
+{
+    equations_average(eqns, "domain", primitive, &forcing_average);
+}
+
+static void forcing_source(void *source_, const double *property, Vector center, double time,
+                           const void *context)
+{
+    (void)property, (void)center, (void)time;
+    NavierStokesConserved *source = source_;
+    const NavierStokesPrimitive *primitive = context;
+    double s1 = primitive->density * (forcing_velocity - forcing_average.velocity.x) /
+                forcing_relaxation;
+    source->momentum.x = s1;
+    source->energy = primitive->velocity.x * s1;
+}
+
+void ref_eqns_set_forcing(Equations *eqns, double velocity, double relaxation)
+{
+    forcing_velocity = velocity;
+    forcing_relaxation = relaxation;
+    equations_set_source(eqns, forcing_source, forcing_prepare);
+}
+
+void ref_average(const Equations *eqns, const char *entity, const double *p, double *avg)
+{
+    equations_average(eqns, entity, p, avg);
+}
+
 double *ref_eqns_primitive(Equations *eqns) { return eqns->primitive.data; }
 double *ref_eqns_conserved(Equations *eqns) { return eqns->conserved.data; }
 double *ref_eqns_eps2(Equations *eqns) { return eqns->limiter.parameter; }

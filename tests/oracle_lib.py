@@ -68,6 +68,10 @@ def oracle():
         _sig(lib, "oracle_eqns_create", VP, VP, C.c_int)
         _sig(lib, "oracle_eqns_destroy", None, VP)
         _sig(lib, "oracle_wtime", D)
+        _sig(lib, "oracle_eqns_set_forcing", None, VP, D, D)
+        _sig(lib, "oracle_average", None, VP, S, DP, DP)
+        _sig(lib, "oracle_norm", None, VP, DP, DP, DP)
+        _sig(lib, "oracle_polar", None, VP, C.c_int, DP, DP, D, DP, DP)
         _bind_common(lib, "oracle_")
         _oracle = lib
     return _oracle
@@ -85,6 +89,8 @@ def ref():
         _sig(lib, "ref_mesh_create", VP, DP, DP, IP, IP)
         _sig(lib, "ref_mesh_validate", None, VP)
         _sig(lib, "ref_eqns_create", VP, VP, C.c_int)
+        _sig(lib, "ref_eqns_set_forcing", None, VP, D, D)
+        _sig(lib, "ref_average", None, VP, S, DP, DP)
         _bind_common(lib, "ref_")
         lib.ref_init()
         _ref = lib

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 from cases import RESIDUAL_CASES, initial_state
-from oracle_lib import Case, arr3, dp, oracle, oracle_eqns, oracle_mesh
+from oracle_lib import Case, arr3, dp, oracle, oracle_eqns, oracle_mesh, random_state
 from teal_amd import _abi
 
 FLUXES = ["godunov", "roe", "hll", "hllc", "hlle", "ausmd", "ausmdv", "lxf", "central"]
@@ -500,3 +500,44 @@ def test_dual_graph_brute_force(shape):
     got = [list(adj[xadj[e]:xadj[e + 1]]) for e in range(ne)]
     assert got == brute_dual(off, idx, 3)
     oracle().oracle_mesh_destroy(m)
+
+
+def test_polar_closed_form():
+    """Uniform pressure p on the "left" side of a box (outward normal -x, the symmetry
+    ghosts keep the pressure): c_x = -p A / (q L) with A the side's area, c_y = 0, and every
+    face's pressure coefficient (p - p_ref) / q (src/euler/polar.c:40-53)."""
+    from oracle_lib import dp as _dp
+
+    case = Case((4, 3, 2), lo=(0, 0, 0), hi=(2.0, 1.5, 0.5), periodic=(0, 0, 0),
+                bcs=[(s, "symmetry", None) for s in ("left", "right", "bottom", "top", "back", "front")])
+    m = oracle_mesh(case)
+    e = oracle_eqns(case, m)
+    a = _abi.mesh_arrays(m)
+    P = np.tile([1.2, 0.3, -0.1, 0.05, 2.5], (a["num_cells"], 1))
+    ref = np.array([1.0, 0.8, 0.6, 0.0, 1.0])
+    q = 1.0 * (0.8 ** 2 + 0.6 ** 2) / 2
+    entity = a["entity_names"].index("left")
+    num = int(a["entity_face_off"][entity + 1] - a["entity_face_off"][entity])
+    coef, pc = np.zeros(2), np.zeros(num)
+    oracle().oracle_polar(e, entity, _dp(P), _dp(ref), 2.0, _dp(coef), _dp(pc))
+    np.testing.assert_allclose(coef, [-2.5 * (1.5 * 0.5) / (q * 2.0), 0.0], rtol=1e-14, atol=1e-15)
+    np.testing.assert_allclose(pc, (2.5 - 1.0) / q, rtol=1e-15)
+
+
+def test_norm_closed_form():
+    """||ref - P|| with ref = P + c per variable is |c| (src/equations/norm.c), and 0 for
+    ref = P."""
+    from oracle_lib import dp as _dp
+
+    case = Case((5, 4, 3))
+    m = oracle_mesh(case)
+    e = oracle_eqns(case, m)
+    a = _abi.mesh_arrays(m)
+    P = random_state(a["num_cells"], 3)
+    ni = a["num_inner"]
+    out = np.zeros(5)
+    oracle().oracle_norm(e, _dp(np.ascontiguousarray(P[:ni])), _dp(P), _dp(out))
+    assert not out.any()
+    shift = np.array([0.5, -0.25, 1e-3, 2.0, -7.0])
+    oracle().oracle_norm(e, _dp(np.ascontiguousarray(P[:ni] + shift)), _dp(P), _dp(out))
+    np.testing.assert_allclose(out, np.abs(shift), rtol=1e-12)

@@ -195,3 +195,67 @@ def test_large_derivative(name, case, kind):
     ref().ref_derivative(er, dp(Pr), dp(Rr))
     np.testing.assert_array_equal(Ro, Rr)
     np.testing.assert_array_equal(Po, Pr)
+
+
+# The source term of the periodic-hill driver (run/periodic_hill/run.c:8-21: a Prepare hook
+# that averages the state over "domain", a per-cell forcing of the x momentum and energy),
+# set on the reference's own Equations through equations_set_source and on the oracle.
+FORCED = [
+    ("ns_channel_forced", Case((6, 5, 4), periodic=(1, 0, 1), kind="navierstokes", mu=0.01,
+                               bcs=[("bottom", "wall", None), ("top", "wall", None)])),
+    ("euler_box_forced", Case((5, 4, 3), flux="roe")),
+]
+
+
+@pytest.mark.parametrize("name,case", FORCED, ids=[c[0] for c in FORCED])
+def test_forcing_source_derivative_and_steps(name, case):
+    m = oracle_mesh(case)
+    eo, er = both(case, m)
+    oracle().oracle_eqns_set_forcing(eo, 1.0, 0.05)
+    ref().ref_eqns_set_forcing(er, 1.0, 0.05)
+    a = _abi.mesh_arrays(m)
+    nc, ni = a["num_cells"], a["num_inner"]
+    P = random_state(nc, 4)
+    P[ni:] = 0
+    Po, Pr = P.copy(), P.copy()
+    Ro, Rr = np.zeros((ni, 5)), np.zeros((ni, 5))
+    oracle().oracle_derivative(eo, dp(Po), dp(Ro))
+    ref().ref_derivative(er, dp(Pr), dp(Rr))
+    np.testing.assert_array_equal(Ro, Rr)
+    # the source changed the residual: the forcing drives <u> towards 1
+    Rn = np.zeros((ni, 5))
+    plain = oracle_eqns(case, m)
+    oracle().oracle_derivative(plain, dp(P.copy()), dp(Rn))
+    assert np.abs(Ro[:, 1] - Rn[:, 1]).min() > 0 and np.array_equal(Ro[:, [0, 2, 3]], Rn[:, [0, 2, 3]])
+    po = np.ctypeslib.as_array(oracle().oracle_eqns_primitive(eo), shape=(nc * 5,))
+    pr = np.ctypeslib.as_array(ref().ref_eqns_primitive(er), shape=(nc * 5,))
+    po[:] = P.ravel()
+    pr[:] = P.ravel()
+    to, tr = C.c_double(0), C.c_double(0)
+    ro, rr = (C.c_double * 5)(), (C.c_double * 5)()
+    for _ in range(2):
+        so = oracle().oracle_lserk(eo, C.byref(to), ro, 1e300, 0.5, 2, 3)
+        sr = ref().ref_lserk(er, C.byref(tr), rr, 1e300, 0.5, 2, 3)
+        assert so == sr and to.value == tr.value and ro[:] == rr[:]
+    np.testing.assert_array_equal(po, pr)
+    for _ in range(2):
+        so = oracle().oracle_implicit_euler(eo, C.byref(to), ro, 1e300, 5.0, 0.1, 0.3, 8)
+        sr = ref().ref_implicit_euler(er, C.byref(tr), rr, 1e300, 5.0, 0.1, 0.3, 8)
+        assert so == sr and to.value == tr.value and ro[:] == rr[:]
+    np.testing.assert_array_equal(po, pr)
+
+
+def test_average_matches_reference():
+    case = Case((7, 5, 3), periodic=(1, 0, 1))
+    m = oracle_mesh(case)
+    eo, er = both(case, m)
+    a = _abi.mesh_arrays(m)
+    P = random_state(a["num_cells"], 11)
+    ao, ar = np.zeros(5), np.zeros(5)
+    oracle().oracle_average(eo, b"domain", dp(P), dp(ao))
+    ref().ref_average(er, b"domain", dp(P), dp(ar))
+    np.testing.assert_array_equal(ao, ar)
+    # closed form: a uniform field averages to itself times (entity volume / mesh volume)
+    U = np.tile([1.0, 2.0, -3.0, 0.5, 7.0], (a["num_cells"], 1))
+    oracle().oracle_average(eo, b"domain", dp(U), dp(ao))
+    np.testing.assert_allclose(ao, U[0], rtol=1e-14)
