@@ -61,6 +61,47 @@ OMesh *oracle_mesh_copy_created(const OMesh *src)
     return m;
 }
 
+/* A created (not yet generated) mesh from plain arrays: nodes, the cell->node graph of the
+ * inner cells followed by one ghost cell per boundary face, and the entities (the inner
+ * entity first, then one boundary entity per ghost block).  No periodic sides, no
+ * neighbour blocks.  For test meshes that mesh_create cannot make (mixed elements). */
+OMesh *oracle_mesh_from_cells(int num_nodes, const double *coord, int num_cells, int num_inner,
+                              const int *node_off, const int *node_idx, int num_entities,
+                              const char *names, const int *entity_cell_off)
+{
+    OMesh *m = o_calloc(1, sizeof(OMesh));
+    m->nodes.num = num_nodes;
+    m->nodes.num_inner = num_nodes;
+    m->nodes.coord = dup(coord, num_nodes * sizeof(OVec));
+    m->nodes.global = o_calloc(num_nodes, sizeof(long));
+    for (int i = 0; i < num_nodes; i++) {
+        m->nodes.global[i] = i;
+    }
+    m->cells.num = num_cells;
+    m->cells.num_inner = num_inner;
+    m->cells.off_boundary = num_cells;
+    m->cells.off_periodic = num_cells;
+    m->cells.node.off = dup(node_off, (num_cells + 1) * sizeof(int));
+    m->cells.node.idx = dup(node_idx, node_off[num_cells] * sizeof(int));
+    m->entities.num = num_entities;
+    m->entities.num_inner = 1;
+    m->entities.off_boundary = num_entities;
+    m->entities.name = o_calloc(num_entities, sizeof(OString));
+    for (int i = 0; i < num_entities; i++) {
+        strncpy(m->entities.name[i], &names[i * sizeof(OString)], sizeof(OString) - 1);
+    }
+    m->entities.cell_off = dup(entity_cell_off, (num_entities + 1) * sizeof(int));
+    m->entities.rotation = o_calloc(num_entities, sizeof(OMat));
+    m->entities.translation = o_calloc(num_entities, sizeof(OVec));
+    m->neighbors.tag = o_calloc(1, sizeof(int[2]));
+    m->neighbors.rank = o_calloc(1, sizeof(int));
+    m->neighbors.recv_off = o_calloc(1, sizeof(int));
+    m->neighbors.recv_off[0] = num_cells;
+    m->neighbors.send.off = o_calloc(1, sizeof(int));
+    m->neighbors.send.idx = o_calloc(1, sizeof(int));
+    return m;
+}
+
 void oracle_mesh_generate(OMesh *mesh) { omesh_generate(mesh); }
 void oracle_mesh_destroy(OMesh *mesh) { omesh_destroy(mesh); }
 int oracle_periodic_source(const OMesh *mesh, int ghost) { return omesh_periodic_source(mesh, ghost); }

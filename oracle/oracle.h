@@ -28,13 +28,16 @@
  *     METIS 5.1.0 CreateGraphDual / FindCommonElements rule; checked by brute
  *     force on small meshes and by the reference's own mesh_validate.
  *   - mesh generation (src/mesh/generate.c needs metis.h, so the reference
- *     build cannot compile it): the reference's mesh_validate, closed-form hex
- *     geometry, closed cells and volume sums on jittered tets, the BFS property
- *     of the cell numbering, the face order.
- *   - parity unpinned: the neighbour order real METIS would return (hence the
- *     BFS tie-breaking and the exact cell/face numbering of mesh_generate)
- *     cannot be compared with METIS itself; it follows the published
- *     algorithm and satisfies every invariant above.
+ *     build cannot compile it): the integer renumbering (connect_cells, the
+ *     breadth-first cell order with its stable sort by descending degree,
+ *     the node order, face creation and the stable face sort) against an
+ *     independent step-by-step restatement on hex boxes and on mixed
+ *     hex/prism meshes, whose cells differ in degree; geometry bit for bit
+ *     against closed forms on boxes where every quantity the reference
+ *     computes is exactly representable, and to 1e-14 on other boxes; closed
+ *     cells and volume sums on jittered tets; the reference's mesh_validate.
+ *   - parity unpinned: only the neighbour order real METIS would return
+ *     (checked against the published rule, not against METIS itself).
  */
 #pragma once
 

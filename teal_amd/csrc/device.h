@@ -174,6 +174,10 @@ void teal_krylov_release(TealGpu *g);
 // synchronization costs tens of microseconds of wake-up per call; the Krylov solver and the
 // single-step entry points synchronize every iteration)
 int teal_stream_wait(TealGpu *g);
+// the exact Riemann solver's error flag of one entry-point call: cleared at its start,
+// read back and turned into an error before it returns (no-ops unless the flux is Godunov)
+int teal_riemann_clear(TealGpu *g);
+int teal_riemann_check(TealGpu *g);
 // poll an event until the work before it has completed
 int teal_event_wait(hipEvent_t ev);
 // residual = sqrt(sum V R^2 / sum V) of a device derivative [5][ni] (src/equations/residual.c)

@@ -9,9 +9,14 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <chrono>
 #include <cstdint>
 #include <string>
+#include <thread>
 #include <vector>
+
+#include <immintrin.h>
+#include <sched.h>
 
 #include "launch.h"
 
@@ -156,7 +161,7 @@ __global__ void k_residual_partials(const double *__restrict__ R, const double *
     double s[5] = {0, 0, 0, 0, 0};
     if (c < ni) {
         for (int v = 0; v < 5; v++) {
-            double r = R[(v * ni) + c];
+            double r = R[((long)v * ni) + c];
             s[v] = vol64[c] * (r * r);
         }
     }
@@ -355,8 +360,11 @@ int DevData<S>::build(const Mesh *m, TealGpu *gpu)
         return cinv[cell];
     };
 
-    if (nc >= (1 << 30)) {  // neighbour entries pack row << 1 | side into an int
-        teal_gpu_set_error("mesh too large for one device handle (%d rows)", nc);
+    // neighbour entries pack row << 1 | side and incidences face << 2 | boundary << 1 | side
+    // into an int; every array index the kernels form from these counts is 64-bit
+    if (nc >= (1 << 30) || nf >= (1 << 29)) {
+        teal_gpu_set_error("mesh too large for one device handle (%d rows, %d faces; the "
+                           "limits are 2^30 rows and 2^29 faces)", nc, nf);
         return -1;
     }
     // incidences: walking faces in reference order keeps each list sorted by reference index
@@ -1135,7 +1143,7 @@ static void fill_periodic_ghosts(TealGpu *g, double *buf, int len)
 template <typename S>
 static int derivative_impl(TealGpu *g, DevData<S> &d, double *primitive, double *derivative)
 {
-    if (upload_primitive(g, d, primitive)) return -1;
+    if (upload_primitive(g, d, primitive) || teal_riemann_clear(g)) return -1;
     if (teal_residual_pass(g, d, 0, 0.0, 0)) return -1;
     // ghost states back to the caller (boundary ghosts computed on the device)
     int nb = g->nfb - g->nfi;
@@ -1149,7 +1157,7 @@ static int derivative_impl(TealGpu *g, DevData<S> &d, double *primitive, double 
     fill_periodic_ghosts(g, primitive, 5);
     if (download_rows(g, (const double *)g->d_R, g->ni, 5, 5, 1, 5, derivative)) return -1;
     g->u0_valid = 0;
-    return 0;
+    return teal_riemann_check(g);
 }
 
 extern "C" int teal_gpu_derivative(TealGpu *g, double *primitive, double *derivative)
@@ -1265,10 +1273,25 @@ extern "C" double teal_gpu_timestep(TealGpu *g, const double *primitive)
     return rc ? NAN : dt;
 }
 
+// Waits of the Krylov iterations and step read-backs last tens of microseconds, where a
+// blocking synchronization's wake-up would dominate: poll, with a pause between queries; after
+// 200 us give the core away between queries, after 2 ms sleep 50 us between them.
 int teal_event_wait(hipEvent_t ev)
 {
     hipError_t e;
+    const auto t0 = std::chrono::steady_clock::now();
     while ((e = hipEventQuery(ev)) == hipErrorNotReady) {
+        const auto us = std::chrono::duration_cast<std::chrono::microseconds>(
+                            std::chrono::steady_clock::now() - t0).count();
+        if (us < 200) {
+            _mm_pause();
+        }
+        else if (us < 2000) {
+            sched_yield();
+        }
+        else {
+            std::this_thread::sleep_for(std::chrono::microseconds(50));
+        }
     }
     HIPCALL(e);
     return 0;
@@ -1417,20 +1440,60 @@ static int explicit_step(TealGpu *g, DevData<S> &d, double max_step, double cour
     return explicit_stages(g, d, max_step, courant, alpha, num_stages);
 }
 
+// The exact Riemann solver's flag (Scalars::error) covers one entry-point call: cleared when
+// the call starts, checked before it returns (the reference stops the program at the first
+// vacuum or non-converged state, src/euler/riemann.c:80,179).  Only Godunov sets it.
+__global__ void k_clear_error(Scalars *scalars, int num)
+{
+    int i = (blockIdx.x * BLOCK) + threadIdx.x;
+    if (i < num) {
+        scalars[i].error = 0;
+    }
+}
+
+int teal_riemann_clear(TealGpu *g)
+{
+    if (g->cfg.flux != TEAL_FLUX_GODUNOV) {
+        return 0;
+    }
+    hipLaunchKernelGGL(k_clear_error, dim3(grid_of(g->ninst)), dim3(BLOCK), 0, g->stream,
+                       g->d_scalars, g->ninst);
+    HIPCALL(hipGetLastError());
+    return 0;
+}
+
+static int scalars_error(TealGpu *g)
+{
+    for (int i = 0; i < g->ninst; i++) {
+        if (g->h_scalars[i].error == 1) {
+            teal_gpu_set_error("vacuum state detected in riemann solver");
+            return -1;
+        }
+        if (g->h_scalars[i].error == 2) {
+            teal_gpu_set_error("newton solver did not converge (20)");
+            return -1;
+        }
+    }
+    return 0;
+}
+
+int teal_riemann_check(TealGpu *g)
+{
+    if (g->cfg.flux != TEAL_FLUX_GODUNOV) {
+        return 0;
+    }
+    HIPCALL(hipMemcpyAsync(g->h_scalars, g->d_scalars, sizeof(Scalars) * (size_t)g->ninst,
+                           hipMemcpyDeviceToHost, g->stream));
+    if (teal_stream_wait(g)) return -1;
+    return scalars_error(g);
+}
+
 static int read_scalars(TealGpu *g)
 {
     HIPCALL(hipMemcpyAsync(g->h_scalars, g->d_scalars, sizeof(Scalars) * (size_t)g->ninst,
                            hipMemcpyDeviceToHost, g->stream));
     if (teal_stream_wait(g)) return -1;
-    if (g->h_scalars->error == 1) {
-        teal_gpu_set_error("vacuum state detected in riemann solver");
-        return -1;
-    }
-    if (g->h_scalars->error == 2) {
-        teal_gpu_set_error("newton solver did not converge (20)");
-        return -1;
-    }
-    return 0;
+    return scalars_error(g);
 }
 
 static int check_single(TealGpu *g, const char *who)
@@ -1452,6 +1515,7 @@ extern "C" double teal_gpu_lserk(TealGpu *g, double *time, double *residual, dou
         return NAN;
     }
     const double *alpha = lserk_alpha[time_order - 1][num_stages - 1];
+    if (teal_riemann_clear(g)) return NAN;
     int rc = (g->precision == TEAL_FP64)
                  ? explicit_step(g, g->d64, max_step, courant, alpha, num_stages)
                  : explicit_step(g, g->d32, max_step, courant, alpha, num_stages);
@@ -1468,6 +1532,7 @@ extern "C" double teal_gpu_euler(TealGpu *g, double *time, double *residual, dou
 {
     if (check_single(g, "euler")) return NAN;
     static const double alpha[2] = {0.0, 1.0};
+    if (teal_riemann_clear(g)) return NAN;
     int rc = (g->precision == TEAL_FP64) ? explicit_step(g, g->d64, max_step, courant, alpha, 1)
                                          : explicit_step(g, g->d32, max_step, courant, alpha, 1);
     if (rc || read_scalars(g)) return NAN;

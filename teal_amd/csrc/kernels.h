@@ -228,10 +228,10 @@ __global__ __launch_bounds__(BLOCK) void k_boundary(int num, int nf, const int *
     }
     int f = bface[i];
     S b9[9];
-    for (int k = 0; k < 9; k++) b9[k] = basis[(k * nf) + f];
+    for (int k = 0; k < 9; k++) b9[k] = basis[((long)k * nf) + f];
     S in[5], out[5], ctx[5];
     load_rec(in, P, bleft[i], 5);
-    for (int v = 0; v < 5; v++) ctx[v] = bctx[(i * 5) + v];
+    for (int v = 0; v < 5; v++) ctx[v] = bctx[((long)i * 5) + v];
     boundary<S>(bkind[i], out, in, (S)gamma, m3(b9), ctx);
     long r = bright[i];
     for (int v = 0; v < 5; v++) P[(r * 5) + v] = out[v];
@@ -269,7 +269,7 @@ __global__ __launch_bounds__(BLOCK) void k_grad_limit(Topo t, Geom<S> g, const S
     // cell's own row and are masked), so a thread waits on memory twice, not twice per face
     int nb[WK];
 #pragma unroll
-    for (int k = 0; k < WK; k++) nb[k] = ld_stream(&t.nbr[(k * ni) + c]);
+    for (int k = 0; k < WK; k++) nb[k] = ld_stream(&t.nbr[((long)k * ni) + c]);
     typedef S S2 __attribute__((ext_vector_type(2)));
     const S2 *wo = reinterpret_cast<const S2 *>(g.w);
     S pn[WK][5], wk[WK][3], ok[WK][3];
@@ -277,14 +277,14 @@ __global__ __launch_bounds__(BLOCK) void k_grad_limit(Topo t, Geom<S> g, const S
     for (int k = 0; k < WK; k++) {
         const long n = (nb[k] < 0) ? c : (nb[k] >> 1);
         load_rec(pn[k], P, n, 5);
-        const S2 a = ld_stream(&wo[((0 * W) + k) * ni + c]);
-        const S2 b = ld_stream(&wo[((1 * W) + k) * ni + c]);
+        const S2 a = ld_stream(&wo[((long)((0 * W) + k) * ni) + c]);
+        const S2 b = ld_stream(&wo[((long)((1 * W) + k) * ni) + c]);
         wk[k][0] = a.x;
         wk[k][1] = a.y;
         wk[k][2] = b.x;
         ok[k][0] = b.y;
         if (LIM != 0) {
-            const S2 e = ld_stream(&wo[((2 * W) + k) * ni + c]);
+            const S2 e = ld_stream(&wo[((long)((2 * W) + k) * ni) + c]);
             ok[k][1] = e.x;
             ok[k][2] = e.y;
         }
@@ -367,8 +367,8 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
         if (g.pairs) {
             typedef float f2 __attribute__((ext_vector_type(2)));
             const f2 *q = reinterpret_cast<const f2 *>(g.pairs);
-            const f2 a = ld_stream(&q[f]), b = ld_stream(&q[nf + f]), c = ld_stream(&q[(2 * nf) + f]),
-                     d = ld_stream(&q[(3 * nf) + f]), e = ld_stream(&q[(4 * nf) + f]);
+            const f2 a = ld_stream(&q[f]), b = ld_stream(&q[(long)nf + f]), c = ld_stream(&q[(2 * (long)nf) + f]),
+                     d = ld_stream(&q[(3 * (long)nf) + f]), e = ld_stream(&q[(4 * (long)nf) + f]);
             area = a.x;
             nrm = v3(a.y, b.x, b.y);
             ol = v3(c.x, c.y, d.x);
@@ -385,7 +385,7 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
             typedef double d2 __attribute__((ext_vector_type(2)));
             const d2 *q = reinterpret_cast<const d2 *>(g.pairs);
             d2 x[7];
-            for (int k = 0; k < 7; k++) x[k] = ld_stream(&q[(k * nf) + f]);
+            for (int k = 0; k < 7; k++) x[k] = ld_stream(&q[((long)k * nf) + f]);
             area = x[0].x;
             bp[0] = x[0].y;
             for (int k = 1; k < 5; k++) {
@@ -408,11 +408,11 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
     if (paired) {
     }
     else if (g.offl) {
-        ol = v3(ld_stream(&g.offl[(0 * nf) + f]), ld_stream(&g.offl[(1 * nf) + f]), ld_stream(&g.offl[(2 * nf) + f]));
-        orr = v3(ld_stream(&g.offr[(0 * nf) + f]), ld_stream(&g.offr[(1 * nf) + f]), ld_stream(&g.offr[(2 * nf) + f]));
+        ol = v3(ld_stream(&g.offl[(0 * (long)nf) + f]), ld_stream(&g.offl[(1 * (long)nf) + f]), ld_stream(&g.offl[(2 * (long)nf) + f]));
+        orr = v3(ld_stream(&g.offr[(0 * (long)nf) + f]), ld_stream(&g.offr[(1 * (long)nf) + f]), ld_stream(&g.offr[(2 * (long)nf) + f]));
         if (g.corr) {
-            cu = v3(ld_stream(&g.corr[(0 * nf) + f]), ld_stream(&g.corr[(1 * nf) + f]), ld_stream(&g.corr[(2 * nf) + f]));
-            cn = ld_stream(&g.corr[(3 * nf) + f]);
+            cu = v3(ld_stream(&g.corr[(0 * (long)nf) + f]), ld_stream(&g.corr[(1 * (long)nf) + f]), ld_stream(&g.corr[(2 * (long)nf) + f]));
+            cn = ld_stream(&g.corr[(3 * (long)nf) + f]);
         }
         else {
             V3<S> dl = v3(ol.x - orr.x, ol.y - orr.y, ol.z - orr.z);
@@ -422,7 +422,7 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
     }
     else {
         const long rr = g.crow[f];
-        V3<S> fc = v3(ld_stream(&g.fcen[(0 * nf) + f]), ld_stream(&g.fcen[(1 * nf) + f]), ld_stream(&g.fcen[(2 * nf) + f]));
+        V3<S> fc = v3(ld_stream(&g.fcen[(0 * (long)nf) + f]), ld_stream(&g.fcen[(1 * (long)nf) + f]), ld_stream(&g.fcen[(2 * (long)nf) + f]));
         V3<S> cl = v3(g.ccen[((long)l * 3) + 0], g.ccen[((long)l * 3) + 1], g.ccen[((long)l * 3) + 2]);
         V3<S> cr = v3(g.ccen[(rr * 3) + 0], g.ccen[(rr * 3) + 1], g.ccen[(rr * 3) + 2]);
         ol = v3(fc.x - cl.x, fc.y - cl.y, fc.z - cl.z);
@@ -453,8 +453,8 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
     }
     if (!paired) {
         area = ld_stream(&g.area[f]);
-        nrm = v3(ld_stream(&g.basis[(0 * nf) + f]), ld_stream(&g.basis[(1 * nf) + f]),
-                 ld_stream(&g.basis[(2 * nf) + f]));
+        nrm = v3(ld_stream(&g.basis[(0 * (long)nf) + f]), ld_stream(&g.basis[(1 * (long)nf) + f]),
+                 ld_stream(&g.basis[(2 * (long)nf) + f]));
     }
     if (VISC) {
         const S mu = (S)ph.mu;
@@ -518,7 +518,7 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
             face_tangents(b9);
         }
         else {
-            for (int i = 0; i < 9; i++) b9[i] = ld_stream(&g.basis[(i * nf) + f]);
+            for (int i = 0; i < 9; i++) b9[i] = ld_stream(&g.basis[((long)i * nf) + f]);
         }
         S flx[5];
         convective<FLUX, S>(flx, ql, qr, gamma, m3(b9), err);
@@ -594,11 +594,11 @@ __device__ inline void stage_epilogue(const Topo &t, const Geom<S> &g, const Phy
             for (int v = 0; v < 5; v++) r[v] = -acc.get(v) * iv;
         }
         if (MODE == 0) {
-            for (int v = 0; v < 5; v++) a.R[(v * ni) + c] = r[v];
+            for (int v = 0; v < 5; v++) a.R[((long)v * ni) + c] = r[v];
         }
         else {
             double step = a.scalars[a.blk_inst ? a.blk_inst[blk] : 0].step;
-            for (int v = 0; v < 5; v++) unew[v] = ld_stream(&a.U0[(v * ni) + c]) + ((r[v] * a.alpha) * step);
+            for (int v = 0; v < 5; v++) unew[v] = ld_stream(&a.U0[((long)v * ni) + c]) + ((r[v] * a.alpha) * step);
             if constexpr (sizeof(S) == 8) {
                 cons2prim<double>(pnew, unew, ph.gamma);
             }
@@ -616,8 +616,8 @@ __device__ inline void stage_epilogue(const Topo &t, const Geom<S> &g, const Phy
             double pst[5];  // the stored state, as the next step's timestep() reads it
             for (int v = 0; v < 5; v++) pst[v] = (double)(S)pnew[v];
             double vol = (double)ld_stream(&g.vol[c]);
-            V3<double> pj = v3((double)ld_stream(&g.proj[c]), (double)ld_stream(&g.proj[ni + c]),
-                                (double)ld_stream(&g.proj[(2 * ni) + c]));
+            V3<double> pj = v3((double)ld_stream(&g.proj[c]), (double)ld_stream(&g.proj[(long)ni + c]),
+                                (double)ld_stream(&g.proj[(2 * (long)ni) + c]));
             dt = ph.navierstokes ? ns_dt<double>(pst, ph.gamma, ph.mu, ph.prandtl, vol, pj)
                                  : euler_dt<double>(pst, ph.gamma, vol, pj);
             double u0n[5];
@@ -628,7 +628,7 @@ __device__ inline void stage_epilogue(const Topo &t, const Geom<S> &g, const Phy
                 prim2cons<double>(u0n, pst, ph.gamma);
             }
             for (int v = 0; v < 5; v++) {
-                __builtin_nontemporal_store(u0n[v], &a.U0_next[(v * ni) + c]);  // read next step
+                __builtin_nontemporal_store(u0n[v], &a.U0_next[((long)v * ni) + c]);  // read next step
                 sums[v] = vol * (r[v] * r[v]);
             }
         }
@@ -671,7 +671,7 @@ __device__ inline void accumulate(const Topo &t, const int *__restrict__ slots, 
     if (VISC) {
 #pragma unroll
         for (int k = 0; k < W; k++) {
-            int e = ld_stream(&slots[(k * ni) + c]);
+            int e = ld_stream(&slots[((long)k * ni) + c]);
             if (e < 0) {
                 continue;
             }
@@ -690,7 +690,7 @@ __device__ inline void accumulate(const Topo &t, const int *__restrict__ slots, 
     if (CONV) {
 #pragma unroll
         for (int k = 0; k < W; k++) {
-            int e = ld_stream(&slots[(k * ni) + c]);
+            int e = ld_stream(&slots[((long)k * ni) + c]);
             if (e < 0) {
                 continue;
             }
@@ -812,14 +812,14 @@ __global__ __launch_bounds__(BLOCK) void k_timestep_u0(Topo t, Geom<S> g, Phys p
         double p[5];
         for (int v = 0; v < 5; v++) p[v] = (double)P[((long)c * 5) + v];
         double vol = (double)ld_stream(&g.vol[c]);
-        V3<double> pj = v3((double)ld_stream(&g.proj[c]), (double)ld_stream(&g.proj[t.ni + c]),
-                            (double)ld_stream(&g.proj[(2 * t.ni) + c]));
+        V3<double> pj = v3((double)ld_stream(&g.proj[c]), (double)ld_stream(&g.proj[(long)t.ni + c]),
+                            (double)ld_stream(&g.proj[(2 * (long)t.ni) + c]));
         dt = ph.navierstokes ? ns_dt<double>(p, ph.gamma, ph.mu, ph.prandtl, vol, pj)
                              : euler_dt<double>(p, ph.gamma, vol, pj);
         if (write_u0) {
             double u[5];
             prim2cons<double>(u, p, ph.gamma);
-            for (int v = 0; v < 5; v++) U0[(v * t.ni) + c] = u[v];
+            for (int v = 0; v < 5; v++) U0[((long)v * t.ni) + c] = u[v];
         }
     }
     __shared__ double red[BLOCK / 64];

@@ -731,7 +731,8 @@ int implicit_step(TealGpu *g, DevData<S> &dm, double *time, double *residual, do
     hipLaunchKernelGGL(k_newton_residual, dim3(nb), dim3(RT), 0, g->stream, k->Rn, (const double *)nullptr,
                        (const double *)nullptr, (const double *)k->D, step, n);
     const int slot = dim + 2;
-    if (two_norms(g, k, k->Rn, k->U, slot)) return -1;
+    // a vacuum in the exact Riemann solver ends the step at the next synchronization
+    if (two_norms(g, k, k->Rn, k->U, slot) || teal_riemann_check(g)) return -1;
     double norm = k->h_scal[slot];
     double cons_norm = k->h_scal[slot + 1];
     // the finite-difference step: the reference's 1e-6 on the FP64 path; on the FP32 path
@@ -751,7 +752,7 @@ int implicit_step(TealGpu *g, DevData<S> &dm, double *time, double *residual, do
         if (residuals()) return -1;
         hipLaunchKernelGGL(k_newton_residual, dim3(nb), dim3(RT), 0, g->stream, k->Rn,
                            (const double *)k->U, (const double *)g->d_U0, (const double *)k->D, step, n);
-        if (two_norms(g, k, k->Rn, k->U, slot)) return -1;
+        if (two_norms(g, k, k->Rn, k->U, slot) || teal_riemann_check(g)) return -1;
         norm = k->h_scal[slot];
         cons_norm = k->h_scal[slot + 1];
         fd_scale = fd_relative * fmax(1, cons_norm) / fmax(1, norm);
@@ -822,10 +823,11 @@ extern "C" double teal_gpu_implicit_euler(TealGpu *g, double *time, double *resi
         return NAN;
     }
     double step0 = NAN;
+    if (teal_riemann_clear(g)) return NAN;
     int rc = (g->precision == TEAL_FP64)
                  ? implicit_step(g, g->d64, time, residual, max_step, courant, newton_tol, krylov_tol,
                                  krylov_dim, &step0)
                  : implicit_step(g, g->d32, time, residual, max_step, courant, newton_tol, krylov_tol,
                                  krylov_dim, &step0);
-    return rc ? NAN : step0;
+    return (rc || teal_riemann_check(g)) ? NAN : step0;
 }

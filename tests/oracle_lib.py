@@ -65,6 +65,8 @@ def oracle():
         _sig(lib, "oracle_mesh_destroy", None, VP)
         _sig(lib, "oracle_periodic_source", C.c_int, VP, C.c_int)
         _sig(lib, "oracle_mesh_to_dual", None, C.c_int, C.c_int, LP, LP, C.c_int, LP, LP, C.c_long)
+        _sig(lib, "oracle_mesh_from_cells", VP, C.c_int, DP, C.c_int, C.c_int, IP, IP, C.c_int,
+             C.c_char_p, IP)
         _sig(lib, "oracle_eqns_create", VP, VP, C.c_int)
         _sig(lib, "oracle_eqns_destroy", None, VP)
         _sig(lib, "oracle_wtime", D)
