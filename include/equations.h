@@ -142,7 +142,7 @@ void equations_residual(const Equations *eqns, const void *derivative, void *res
 void equations_average(const Equations *eqns, const char *entity, const void *primitive,
                        void *average);                                          // :181
 void equations_norm(const Equations *eqns, void *norm, double time);            // :184
-// VTKHDF output / restart (src/equations.h:187-190): not provided by this library (teal_error).
+// VTKHDF output and nearest-centre restart (src/equations.h:187-190), serial HDF5.
 void equations_write(const Equations *eqns, const char *name, double time, int index);
 void equations_read(const Equations *eqns, const char *fname, double *time, int *index);
 void equations_destroy(Equations *eqns);                                        // :194

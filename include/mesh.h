@@ -104,7 +104,7 @@ void mesh_validate(const Mesh *mesh);
 // Print a mesh summary (src/mesh.h:93, src/mesh/summary.c:7-29).
 void mesh_summary(const Mesh *mesh);
 
-// VTKHDF output (src/mesh.h:96): not provided by this library; reports an error.
+// VTKHDF output (src/mesh.h:96, src/mesh/write.c:185-211): <name>_mesh.hdf.
 void mesh_write(const Mesh *mesh, const char *name);
 
 // Release all mesh resources (src/mesh.h:99).
@@ -114,6 +114,11 @@ void mesh_destroy(Mesh *mesh);
 // files): an n^3 hex grid on [0,1]^3, each hex split into 6 tets, interior nodes jittered
 // by +-jitter*h with a seeded generator; boundary sides named as in mesh_create.
 Mesh *mesh_create_tets(int n, double jitter, unsigned long long seed);
+
+// Source cell of each periodic ghost of a one-process mesh (the exchange pairing of
+// src/equations/exchange.c:19-33): src[ghost] = source cell for every ghost that receives one;
+// other entries are left unchanged.  No counterpart in the reference (its exchange is MPI).
+void teal_mesh_periodic_sources(const Mesh *mesh, int *src);
 
 #ifdef __cplusplus
 }

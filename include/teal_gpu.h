@@ -112,6 +112,33 @@ double teal_gpu_implicit_euler(TealGpu *gpu, double *time, double *residual, dou
 enum TealReduction { TEAL_REDUCE_TREE = 0, TEAL_REDUCE_SERIAL = 1 };
 int teal_gpu_set_reductions(TealGpu *gpu, int mode);
 
+// Source term on the device (no counterpart in the reference's library, whose sources are user
+// callbacks, src/equations.h:27-29, 41): TEAL_SOURCE_FORCING is the bulk-velocity forcing of
+// run/periodic_hill/run.c:8-21.  Before every residual evaluation its Prepare step averages the
+// state being evaluated over inner entity `entity` (src/equations/average.c:8-41); the
+// finalize step then adds s1 = rho (U - <u>) / tau to the x momentum and u s1 to the energy of
+// each cell (src/equations/derivative.c:260-265), param = {U, tau}.  TEAL_SOURCE_NONE removes
+// it.  Single-mesh handles only.
+enum TealSource { TEAL_SOURCE_NONE = 0, TEAL_SOURCE_FORCING = 1 };
+int teal_gpu_set_source(TealGpu *gpu, int kind, int entity, const double *param);
+
+// Diagnostics as device reductions (single-mesh handles):
+// teal_gpu_average -- src/equations/average.c:8-41: volume-weighted sum over inner entity
+//   `entity` divided by the mesh volume; primitive = host array in the reference layout, or
+//   null for the resident state.
+// teal_gpu_norm -- src/equations/norm.c:8-38: sqrt(sum V (reference - primitive)^2 / sum V)
+//   over the inner cells, both host arrays [num_inner][5] in the reference layout.
+// teal_gpu_polar -- the face loop of src/euler/polar.c:36-53 on boundary entity `entity`:
+//   the boundary pass on primitive (host array, its ghost rows updated as the reference's
+//   boundary pass does; null: the resident state), then pressure_c[j] per face (may be null)
+//   and the force coefficients coef[0..1] summed in face order; reference = the free-stream
+//   primitive state, length = the reference length.
+// The cell sums use the reduction mode of teal_gpu_set_reductions (serial: bit for bit).
+int teal_gpu_average(TealGpu *gpu, int entity, const double *primitive, double *average);
+int teal_gpu_norm(TealGpu *gpu, const double *primitive, const double *reference, double *norm);
+int teal_gpu_polar(TealGpu *gpu, int entity, double *primitive, const double *reference,
+                   double length, double *coef, double *pressure_c);
+
 // Iteration counts of the last teal_gpu_implicit_euler call: stats[0] Newton iterations,
 // stats[1] GMRES iterations (one matvec, i.e. one residual evaluation, each).  No counterpart
 // in the reference, which does not report them.
@@ -175,6 +202,18 @@ int teal_gpu_info(TealGpu *gpu, long *counts /* [8]: ni, nc, nf, nfi, nfb, W, fu
 // next device call; the mirror is rebuilt when the precision changes.
 struct Equations;
 void teal_equations_set_precision(struct Equations *eqns, int precision);
+
+// The bulk-velocity forcing source (run/periodic_hill/run.c:8-21) for the teal API: sets the
+// equations' source to the pair teal_forcing_source / teal_forcing_prepare, which the device
+// path runs as TEAL_SOURCE_FORCING with the average over inner entity `entity`, target
+// velocity `velocity` and relaxation time `relaxation`.  The pair are ordinary callbacks with
+// the reference signatures (src/equations.h:27-29, 41) for host callers; other source
+// callbacks have no device counterpart and are refused (teal_error) when the device is used.
+void teal_equations_set_forcing(struct Equations *eqns, const char *entity, double velocity,
+                                double relaxation);
+void teal_forcing_source(void *source, const double *property, Vector center, double time,
+                         const void *context);
+void teal_forcing_prepare(const struct Equations *eqns, const void *primitive);
 
 // The configured device mirror of an Equations (created on first use; teal_error when one of
 // its plugins has no device counterpart).  For callers that drive the C-ABI above directly.

@@ -21,6 +21,7 @@ LP = C.POINTER(C.c_long)
 VP = C.c_void_p
 
 FP64, MIXED = 0, 1
+SOURCE_NONE, SOURCE_FORCING = 0, 1
 REDUCE_TREE, REDUCE_SERIAL = 0, 1
 FLUX = {"none": 0, "godunov": 1, "roe": 2, "hll": 3, "hllc": 4, "hlle": 5, "ausmd": 6,
         "ausmdv": 7, "lxf": 8, "central": 9}
@@ -96,6 +97,10 @@ def lib():
         _sig(l, "teal_gpu_run_lserk_batch", C.c_int, VP, C.c_int, DP, DP, D, C.c_int, C.c_int)
         _sig(l, "teal_gpu_set_graphs", C.c_int, VP, C.c_int)
         _sig(l, "teal_gpu_autotune", C.c_int, VP, C.c_int, D, C.c_int, C.c_int, DP)
+        _sig(l, "teal_gpu_set_source", C.c_int, VP, C.c_int, C.c_int, DP)
+        _sig(l, "teal_gpu_average", C.c_int, VP, C.c_int, DP, DP)
+        _sig(l, "teal_gpu_norm", C.c_int, VP, DP, DP, DP)
+        _sig(l, "teal_gpu_polar", C.c_int, VP, C.c_int, DP, DP, D, DP, DP)
         _lib = l
     return _lib
 
@@ -216,6 +221,37 @@ class Gpu:
         check(self.lib.teal_gpu_autotune(self.h, steps, courant, time_order, num_stages, dp(ms)),
               "autotune")
         return ms
+
+    def set_source(self, kind, entity=0, params=(0.0, 1.0)):
+        """Bulk-velocity forcing (kind SOURCE_FORCING): params = (U, tau), averaged over inner
+        entity `entity`; SOURCE_NONE removes it."""
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        check(self.lib.teal_gpu_set_source(self.h, kind, entity, dp(p)), "set_source")
+        return self
+
+    def average(self, entity=0, P=None):
+        out = np.zeros(5)
+        src = None if P is None else dp(np.ascontiguousarray(P, dtype=np.float64))
+        check(self.lib.teal_gpu_average(self.h, entity, src, dp(out)), "average")
+        return out
+
+    def norm(self, P, ref):
+        out = np.zeros(5)
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        ref = np.ascontiguousarray(ref, dtype=np.float64)
+        check(self.lib.teal_gpu_norm(self.h, dp(P), dp(ref), dp(out)), "norm")
+        return out
+
+    def polar(self, entity, P, reference, length):
+        """(coef[2], pressure coefficient per face, P with its ghost rows updated)"""
+        P = np.ascontiguousarray(P, dtype=np.float64).copy()
+        ref = np.ascontiguousarray(reference, dtype=np.float64)
+        off = self.arrays["entity_face_off"]
+        pc = np.zeros(max(int(off[entity + 1] - off[entity]), 1))
+        coef = np.zeros(2)
+        check(self.lib.teal_gpu_polar(self.h, entity, dp(P), dp(ref), length, dp(coef), dp(pc)),
+              "polar")
+        return coef, pc[:int(off[entity + 1] - off[entity])], P
 
     def krylov_stats(self):
         st = (C.c_long * 2)()

@@ -27,6 +27,11 @@ build_hip mixed.hip -ffp-contract=fast -fno-hip-fp32-correctly-rounded-divide-sq
     -fgpu-flush-denormals-to-zero & pids+=($!)
 build_hip krylov.hip -ffp-contract=off & pids+=($!)
 build_hip plugins.hip -ffp-contract=off & pids+=($!)
+build_hip diagnostics.hip -ffp-contract=off & pids+=($!)
+# VTKHDF output and restart: the image's serial HDF5 headers (/opt/conda); the library itself
+# is loaded at run time (io.c)
+H5DIR="${TEAL_HDF5:-/opt/conda}"
+gcc "${CFLAGS[@]}" -I"$H5DIR/include" -c "$HERE/io.c" -o "$OBJ/io.o" & pids+=($!)
 for src in core.c sync.c mesh_create.c mesh_generate.c mesh_util.c mesh_union.c equations.c simulation.c; do
     if [ -f "$HERE/$src" ]; then
         gcc "${CFLAGS[@]}" -c "$HERE/$src" -o "$OBJ/${src%.*}.o" & pids+=($!)
@@ -35,6 +40,6 @@ done
 rc=0
 for p in "${pids[@]}"; do wait "$p" || rc=1; done
 [ $rc -eq 0 ] || { echo "build failed" >&2; exit 1; }
-/opt/rocm/bin/hipcc -shared -fPIC --offload-arch="$ARCH" -o "$OUT.tmp" "$OBJ"/*.o -lm -Wl,--no-undefined
+/opt/rocm/bin/hipcc -shared -fPIC --offload-arch="$ARCH" -o "$OUT.tmp" "$OBJ"/*.o -lm -ldl -Wl,--no-undefined
 mv "$OUT.tmp" "$OUT"
 echo "built $OUT"

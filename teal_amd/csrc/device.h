@@ -158,7 +158,18 @@ struct TealGpu {
         const void *P;
     } graph_key{};
     Krylov *krylov = nullptr;
-    int serial_reductions = 0;  // TEAL_REDUCE_SERIAL: GMRES/Newton dot products in reference order
+    int serial_reductions = 0;  // TEAL_REDUCE_SERIAL: GMRES/Newton dot products, entity averages
+                                // and norms in reference order
+    // source term (teal_gpu_set_source): its kind, the inner entity its average runs over,
+    // and {average (5), velocity, relaxation} on the device for the stage epilogue
+    int source = 0;
+    int source_entity = 0;
+    double *d_forcing = nullptr;
+    // diagnostics (diagnostics.hip): partial sums, the device row of each mesh inner cell
+    // (reference order) and the last result
+    double2 *d_cellpart = nullptr;
+    int *d_cell_inv = nullptr;
+    double *d_diag = nullptr;
 
     fv::Topo topo() const;
     fv::Phys phys() const;
@@ -170,6 +181,15 @@ int teal_residual_pass(TealGpu *g, DevData<S> &d, int mode, double alpha, int la
 template <typename S>
 int teal_timestep_pass(TealGpu *g, DevData<S> &d, int write_u0, double *out);
 void teal_krylov_release(TealGpu *g);
+void teal_diagnostics_release(TealGpu *g);
+// average of the forcing source's entity over the state P into d_forcing (no-op without a
+// source): the Prepare step of every residual evaluation
+template <typename S>
+int teal_prepare_source(TealGpu *g, const S *P);
+// host primitive (reference layout, num_cells x 5) -> the resident state; boundary-ghost rows
+// of the resident state -> the host array
+int teal_upload_state(TealGpu *g, const double *primitive);
+int teal_download_ghosts(TealGpu *g, double *primitive);
 // wait for the handle's stream by polling an event recorded on it (a blocking
 // synchronization costs tens of microseconds of wake-up per call; the Krylov solver and the
 // single-step entry points synchronize every iteration)

@@ -228,26 +228,11 @@ static int upload_conv(S **dst, const std::vector<double> &src)
     return upload(dst, tmp);
 }
 
-// the ghost block of neighbour entry e receives the send list of the entry whose send tag is
-// e's receive tag (src/equations/exchange.c:19-33, src/mesh/create.c:409-419)
+// source cell of each periodic ghost (-1 for other cells), src/mesh/create.c:409-419
 static std::vector<int> periodic_sources(const Mesh *m)
 {
     std::vector<int> src(m->cells.num, -1);
-    const MeshNeighbors *nb = &m->neighbors;
-    for (int e = 0; e < nb->num; e++) {
-        int s = -1;
-        for (int k = 0; k < nb->num; k++) {
-            if (nb->tag[k][1] == nb->tag[e][0]) {
-                s = k;
-            }
-        }
-        if (s < 0) {
-            continue;
-        }
-        for (int j = nb->recv_off[e]; j < nb->recv_off[e + 1]; j++) {
-            src[j] = nb->send.idx[nb->send.off[s] + (j - nb->recv_off[e])];
-        }
-    }
+    teal_mesh_periodic_sources(m, src.data());
     return src;
 }
 
@@ -864,6 +849,7 @@ extern "C" void teal_gpu_destroy(TealGpu *g)
     g->ev_grad.release();
     g->ev_face.release();
     teal_krylov_release(g);
+    teal_diagnostics_release(g);
     if (g->ev_wait) {
         (void)hipEventDestroy(g->ev_wait);
     }
@@ -1020,8 +1006,10 @@ int teal_residual_pass(TealGpu *g, DevData<S> &d, int mode, double alpha, int la
             g->ev_grad.end(g->stream);
         }
     }
+    if (teal_prepare_source(g, (const S *)d.d_P)) return -1;
     const int conv = g->cfg.flux != TEAL_FLUX_NONE;
     StageArgs a;
+    a.forcing = g->source ? g->d_forcing : nullptr;
     a.R = g->d_R;
     a.U0 = g->d_U0;
     a.U0_next = g->d_U0;
@@ -1140,20 +1128,41 @@ static void fill_periodic_ghosts(TealGpu *g, double *buf, int len)
     }
 }
 
+// boundary-ghost rows of the device state -> the caller's host array (the rows the reference's
+// boundary pass writes, src/equations/boundary.c:5-43)
+template <typename S>
+static int download_ghosts(TealGpu *g, DevData<S> &d, double *primitive)
+{
+    if (g->nfb - g->nfi <= 0) {
+        return 0;
+    }
+    std::vector<double> tmp((size_t)5 * g->mesh->cells.num);
+    if (download_rows(g, d.d_P, g->nc, 5, 5, 0, 5, tmp.data())) return -1;
+    for (int c = g->mesh_ni; c < g->mesh->cells.off_boundary; c++) {
+        memcpy(&primitive[(size_t)c * 5], &tmp[(size_t)c * 5], sizeof(double) * 5);
+    }
+    return 0;
+}
+
+int teal_upload_state(TealGpu *g, const double *primitive)
+{
+    return (g->precision == TEAL_FP64) ? upload_primitive(g, g->d64, primitive)
+                                       : upload_primitive(g, g->d32, primitive);
+}
+
+int teal_download_ghosts(TealGpu *g, double *primitive)
+{
+    return (g->precision == TEAL_FP64) ? download_ghosts(g, g->d64, primitive)
+                                       : download_ghosts(g, g->d32, primitive);
+}
+
 template <typename S>
 static int derivative_impl(TealGpu *g, DevData<S> &d, double *primitive, double *derivative)
 {
     if (upload_primitive(g, d, primitive) || teal_riemann_clear(g)) return -1;
     if (teal_residual_pass(g, d, 0, 0.0, 0)) return -1;
     // ghost states back to the caller (boundary ghosts computed on the device)
-    int nb = g->nfb - g->nfi;
-    if (nb > 0) {
-        std::vector<double> tmp((size_t)5 * g->mesh->cells.num);
-        if (download_rows(g, d.d_P, g->nc, 5, 5, 0, 5, tmp.data())) return -1;
-        for (int c = g->mesh_ni; c < g->mesh->cells.off_boundary; c++) {
-            memcpy(&primitive[(size_t)c * 5], &tmp[(size_t)c * 5], sizeof(double) * 5);
-        }
-    }
+    if (download_ghosts(g, d, primitive)) return -1;
     fill_periodic_ghosts(g, primitive, 5);
     if (download_rows(g, (const double *)g->d_R, g->ni, 5, 5, 1, 5, derivative)) return -1;
     g->u0_valid = 0;
@@ -1175,11 +1184,7 @@ static int boundary_impl(TealGpu *g, DevData<S> &d, double *primitive)
     if (nb > 0) {
         Kern<S>::boundary(g->stream, nb, g->nf, d, g->cfg.gamma);
         HIPCALL(hipGetLastError());
-        std::vector<double> tmp((size_t)5 * g->mesh->cells.num);
-        if (download_rows(g, d.d_P, g->nc, 5, 5, 0, 5, tmp.data())) return -1;
-        for (int c = g->mesh_ni; c < g->mesh->cells.off_boundary; c++) {
-            memcpy(&primitive[(size_t)c * 5], &tmp[(size_t)c * 5], sizeof(double) * 5);
-        }
+        if (download_ghosts(g, d, primitive)) return -1;
     }
     g->u0_valid = 0;
     return 0;

@@ -15,57 +15,78 @@
 #include "teal.h"
 #include "teal_gpu.h"
 
+// ------------------------------------------------------------------ helpers
+// index of `name` among names[first, last), or -1
+static int find_name(const String *names, int first, int last, const char *name)
+{
+    for (int i = first; i < last; i++) {
+        if (strcmp(names[i], name) == 0) {
+            return i;
+        }
+    }
+    return -1;
+}
+
+// a String array of `num` names, each formatted from name[i] with `fmt` ("%s", "%s-ref", ...)
+static String *name_list(const char *const *name, int num, const char *fmt)
+{
+    String *out = teal_calloc(num, sizeof(String));
+    for (int i = 0; i < num; i++) {
+        snprintf(out[i], sizeof(String), fmt, name[i]);
+    }
+    return out;
+}
+
+// a variable set: names, per-variable dimensions, their sum (the stride) and one zeroed row of
+// `stride` doubles per mesh cell (src/equations/create.c:33-50: ghost rows included)
+static void variable_set(EquationsVariables *var, String *names, const int *dimension, int num,
+                         const Mesh *mesh)
+{
+    var->num = num;
+    var->name = names;
+    var->dimension = teal_calloc(num, sizeof(int));
+    var->stride = 0;
+    for (int i = 0; i < num; i++) {
+        assert(dimension[i] > 0);
+        var->dimension[i] = dimension[i];
+        var->stride += dimension[i];
+    }
+    var->data = teal_calloc((size_t)mesh->cells.num * var->stride, sizeof(double));
+}
+
+static const char *entity_name(const Equations *eqns, int entity)
+{
+    return eqns->mesh->entities.name[entity];
+}
+
 // ------------------------------------------------------------------ creation
 Equations *equations_create(const Mesh *mesh, const char *name, Timestep *timestep,
                             ConvectiveSelect *convective, ViscousSelect *viscous,
                             BoundarySelect *boundary)
 {
     assert(mesh && name && timestep);
-    Equations *eqns = teal_calloc(1, sizeof(*eqns));
-    eqns->mesh = mesh;
-    strcpy(eqns->name, name);
+    Equations *eqns = teal_calloc(1, sizeof(Equations));
+    *eqns = (Equations){.mesh = mesh};
+    snprintf(eqns->name, sizeof(String), "%s", name);
     eqns->timestep.compute = timestep;
     eqns->convective.select = convective;
     eqns->viscous.select = viscous;
-    eqns->boundary.select = boundary;
-    int num = mesh->entities.off_boundary - mesh->entities.num_inner;
-    eqns->boundary.num = num;
-    eqns->boundary.name = teal_calloc(num, sizeof(String));
-    eqns->boundary.context = teal_calloc(num, sizeof(*eqns->boundary.context));
-    eqns->boundary.compute = teal_calloc(num, sizeof(*eqns->boundary.compute));
-    eqns->boundary.condition = teal_calloc(num, sizeof(*eqns->boundary.condition));
+    // one slot per boundary entity (the entities between num_inner and off_boundary)
+    EquationsBoundary *bc = &eqns->boundary;
+    bc->select = boundary;
+    bc->num = mesh->entities.off_boundary - mesh->entities.num_inner;
+    bc->name = teal_calloc(bc->num, sizeof(String));
+    bc->compute = teal_calloc(bc->num, sizeof(Compute *));
+    bc->condition = teal_calloc(bc->num, sizeof(Boundary *));
+    bc->context = teal_calloc(bc->num, sizeof(void *));
     return eqns;
-}
-
-static void create_variables(EquationsVariables *var, const Equations *eqns, const int *dim,
-                             int num)
-{
-    int stride = 0;
-    for (int i = 0; i < num; i++) {
-        assert(dim[i] > 0);
-        stride += dim[i];
-    }
-    var->num = num;
-    var->stride = stride;
-    var->dimension = teal_calloc(num, sizeof(int));
-    memcpy(var->dimension, dim, num * sizeof(int));
-    var->data = teal_calloc(eqns->mesh->cells.num, sizeof(double) * stride);
-}
-
-static void copy_names(String **dst, const char **name, int num)
-{
-    *dst = teal_calloc(num, sizeof(String));
-    for (int i = 0; i < num; i++) {
-        strcpy((*dst)[i], name[i]);
-    }
 }
 
 void equations_create_primitive(Equations *eqns, const char **name, const int *dimension,
                                 Convert *convert, int num)
 {
     assert(eqns && name && dimension && num > 0);
-    copy_names(&eqns->primitive.name, name, num);
-    create_variables(&eqns->primitive, eqns, dimension, num);
+    variable_set(&eqns->primitive, name_list(name, num, "%s"), dimension, num, eqns->mesh);
     eqns->primitive.func.convert = convert;
 }
 
@@ -73,20 +94,22 @@ void equations_create_conserved(Equations *eqns, const char **name, const int *d
                                 Convert *convert, int num)
 {
     assert(eqns && name && dimension && num > 0);
-    copy_names(&eqns->conserved.name, name, num);
-    create_variables(&eqns->conserved, eqns, dimension, num);
+    variable_set(&eqns->conserved, name_list(name, num, "%s"), dimension, num, eqns->mesh);
     eqns->conserved.func.convert = convert;
 }
 
+// the reference solution mirrors the primitive variables, named "<primitive>-ref"
 void equations_create_reference(Equations *eqns, Compute *compute)
 {
     assert(eqns && compute);
-    int num = eqns->primitive.num;
-    eqns->reference.name = teal_calloc(num, sizeof(String));
-    for (int i = 0; i < num; i++) {
-        snprintf(eqns->reference.name[i], sizeof(String), "%s-ref", eqns->primitive.name[i]);
+    const EquationsVariables *prim = &eqns->primitive;
+    const char *names[64];
+    assert(prim->num <= 64);
+    for (int i = 0; i < prim->num; i++) {
+        names[i] = prim->name[i];
     }
-    create_variables(&eqns->reference, eqns, eqns->primitive.dimension, num);
+    variable_set(&eqns->reference, name_list(names, prim->num, "%s-ref"), prim->dimension,
+                 prim->num, eqns->mesh);
     eqns->reference.func.compute = compute;
 }
 
@@ -94,159 +117,150 @@ void equations_create_properties(Equations *eqns, const char **name, const doubl
                                  int num)
 {
     assert(eqns && name && property && num > 0);
-    eqns->properties.num = num;
-    copy_names(&eqns->properties.name, name, num);
-    eqns->properties.data = teal_calloc(num, sizeof(double));
-    memcpy(eqns->properties.data, property, num * sizeof(double));
+    EquationsProperties *prop = &eqns->properties;
+    prop->num = num;
+    prop->name = name_list(name, num, "%s");
+    prop->data = teal_calloc(num, sizeof(double));
+    memcpy(prop->data, property, sizeof(double) * num);
 }
 
 // ------------------------------------------------------------------ setters
 void equations_set_space_order(Equations *eqns, int space_order)
 {
-    assert(eqns && 1 <= space_order && space_order <= 2);
+    assert(eqns && (space_order == 1 || space_order == 2));
     eqns->space_order = space_order;
 }
 
+// a plugin chosen by name: the name is kept for the summary; null clears both
 void equations_set_convective_flux(Equations *eqns, const char *name)
 {
     assert(eqns);
-    if (!name) {
-        eqns->convective.name[0] = 0;
-        eqns->convective.flux = 0;
-        return;
-    }
-    strcpy(eqns->convective.name, name);
-    eqns->convective.flux = eqns->convective.select(name);
+    snprintf(eqns->convective.name, sizeof(String), "%s", name ? name : "");
+    eqns->convective.flux = name ? eqns->convective.select(name) : 0;
 }
 
 void equations_set_viscous_flux(Equations *eqns, const char *name)
 {
     assert(eqns);
-    if (!name) {
-        eqns->viscous.name[0] = 0;
-        eqns->viscous.flux = 0;
-        return;
-    }
-    strcpy(eqns->viscous.name, name);
-    eqns->viscous.flux = eqns->viscous.select(name);
+    snprintf(eqns->viscous.name, sizeof(String), "%s", name ? name : "");
+    eqns->viscous.flux = name ? eqns->viscous.select(name) : 0;
 }
 
-// Venkatakrishnan parameter eps^2 = (K cbrt(V))^3 per inner cell (src/equations/limiter.c:8-21)
-static double *venkatakrishnan_parameter(const Equations *eqns, double coefficient)
-{
-    int ni = eqns->mesh->cells.num_inner;
-    double *eps2 = teal_calloc(ni, sizeof(double));
-    for (int i = 0; i < ni; i++) {
-        eps2[i] = cb(coefficient * cbrt(eqns->mesh->cells.volume[i]));
-    }
-    return eps2;
-}
+// limiter names (src/equations/set.c:52-68): minmod is Barth-Jespersen
+static const struct {
+    const char *name;
+    int kind;
+} limiter_names[] = {
+    {"barth jespersen", BARTH_JESPERSEN},
+    {"minmod", BARTH_JESPERSEN},
+    {"venkatakrishnan", VENKATAKRISHNAN},
+};
 
 void equations_set_limiter(Equations *eqns, const char *name, double coefficient)
 {
     assert(eqns);
-    teal_free(eqns->limiter.parameter);
-    eqns->limiter.parameter = 0;
+    EquationsLimiter *lim = &eqns->limiter;
+    teal_free(lim->parameter);
+    lim->parameter = 0;
+    lim->kind = NONE;
+    snprintf(lim->name, sizeof(String), "%s", name ? name : "");
     if (!name) {
-        eqns->limiter.name[0] = 0;
-        eqns->limiter.kind = NONE;
         return;
     }
-    strcpy(eqns->limiter.name, name);
-    if (!strcmp(name, "barth jespersen") || !strcmp(name, "minmod")) {
-        eqns->limiter.kind = BARTH_JESPERSEN;
+    for (size_t i = 0; i < sizeof(limiter_names) / sizeof(limiter_names[0]); i++) {
+        if (strcmp(limiter_names[i].name, name) == 0) {
+            lim->kind = limiter_names[i].kind;
+        }
     }
-    else if (!strcmp(name, "venkatakrishnan")) {
-        eqns->limiter.kind = VENKATAKRISHNAN;
-        eqns->limiter.parameter = venkatakrishnan_parameter(eqns, coefficient);
-    }
-    else {
+    if (lim->kind == NONE) {
         teal_error("invalid limiter (%s)", name);
+    }
+    if (lim->kind == VENKATAKRISHNAN) {
+        // eps^2 = (K cbrt(V))^3 per inner cell (src/equations/limiter.c:8-21)
+        const int ni = eqns->mesh->cells.num_inner;
+        lim->parameter = teal_calloc(ni, sizeof(double));
+        for (int i = 0; i < ni; i++) {
+            lim->parameter[i] = cb(coefficient * cbrt(eqns->mesh->cells.volume[i]));
+        }
     }
 }
 
+// A named condition from the equations' table, or a user callback ("custom" or no name); the
+// context is borrowed (it must outlive the equations).
 void equations_set_boundary_condition(Equations *eqns, const char *entity, const char *name,
                                       Compute *compute, const void *context)
 {
     assert(eqns && entity);
-    int num_inner = eqns->mesh->entities.num_inner;
-    for (int i = 0; i < eqns->boundary.num; i++) {
-        if (!strcmp(eqns->mesh->entities.name[num_inner + i], entity)) {
-            if (!name || !strcmp(name, "custom")) {
-                assert(compute);
-                strcpy(eqns->boundary.name[i], "custom");
-                eqns->boundary.compute[i] = compute;
-                eqns->boundary.condition[i] = 0;
-            }
-            else {
-                strcpy(eqns->boundary.name[i], name);
-                eqns->boundary.compute[i] = 0;
-                eqns->boundary.condition[i] = eqns->boundary.select(name);
-            }
-            eqns->boundary.context[i] = context;
-            return;
-        }
+    const Mesh *mesh = eqns->mesh;
+    int e = find_name(mesh->entities.name, mesh->entities.num_inner, mesh->entities.off_boundary,
+                      entity);
+    if (e < 0) {
+        teal_error("invalid entity (%s)", entity);
     }
-    teal_error("invalid entity (%s)", entity);
+    const int slot = e - mesh->entities.num_inner;
+    const int custom = !name || strcmp(name, "custom") == 0;
+    assert(!custom || compute);
+    EquationsBoundary *bc = &eqns->boundary;
+    snprintf(bc->name[slot], sizeof(String), "%s", custom ? "custom" : name);
+    bc->compute[slot] = custom ? compute : 0;
+    bc->condition[slot] = custom ? 0 : bc->select(name);
+    bc->context[slot] = context;
 }
 
+// every cell of an inner entity from a callback of its centre, or a copy of one state
 void equations_set_initial_condition(Equations *eqns, const char *entity, Compute *compute,
                                      const void *initial)
 {
     assert(eqns && entity);
-    int num_inner = eqns->mesh->entities.num_inner;
-    int *cell_off = eqns->mesh->entities.cell_off;
-    int stride = eqns->primitive.stride;
-    double *primitive = eqns->primitive.data;
-    for (int i = 0; i < num_inner; i++) {
-        if (!strcmp(eqns->mesh->entities.name[i], entity)) {
-            for (int j = cell_off[i]; j < cell_off[i + 1]; j++) {
-                double *p = &primitive[(size_t)j * stride];
-                if (compute) {
-                    compute(p, eqns->properties.data, eqns->mesh->cells.center[j], 0, 0);
-                }
-                else {
-                    memcpy(p, initial, stride * sizeof(double));
-                }
-            }
-            return;
+    const Mesh *mesh = eqns->mesh;
+    int e = find_name(mesh->entities.name, 0, mesh->entities.num_inner, entity);
+    if (e < 0) {
+        teal_error("invalid entity (%s)", entity);
+    }
+    const int stride = eqns->primitive.stride;
+    double *rows = eqns->primitive.data;
+    for (int c = mesh->entities.cell_off[e]; c < mesh->entities.cell_off[e + 1]; c++) {
+        double *row = &rows[(size_t)c * stride];
+        if (compute) {
+            compute(row, eqns->properties.data, mesh->cells.center[c], 0, 0);
+        }
+        else {
+            memcpy(row, initial, sizeof(double) * stride);
         }
     }
-    teal_error("invalid entity (%s)", entity);
 }
 
 void equations_set_property(Equations *eqns, const char *name, double property)
 {
     assert(eqns && name);
-    for (int i = 0; i < eqns->properties.num; i++) {
-        if (!strcmp(eqns->properties.name[i], name)) {
-            eqns->properties.data[i] = property;
-            return;
-        }
+    int i = find_name(eqns->properties.name, 0, eqns->properties.num, name);
+    if (i < 0) {
+        teal_error("invalid property (%s)", name);
     }
-    teal_error("invalid property (%s)", name);
+    eqns->properties.data[i] = property;
 }
 
 void equations_set_source(Equations *eqns, Compute *compute, Prepare *prepare)
 {
     assert(eqns && compute);
-    eqns->source.compute = compute;
-    eqns->source.prepare = prepare;
+    eqns->source = (EquationsSource){.compute = compute, .prepare = prepare};
 }
 
 void equations_summary(const Equations *eqns)
 {
     assert(eqns);
-    int num_inner = eqns->mesh->entities.num_inner;
+    const char *label[] = {"name", "space order", "convective flux", "viscous flux", "limiter"};
+    char order[16];
+    snprintf(order, sizeof(order), "%d", eqns->space_order);
+    const char *value[] = {eqns->name, order, eqns->convective.name, eqns->viscous.name,
+                           eqns->limiter.name};
     teal_print("Equations summary");
-    teal_print("\t name            : %s", eqns->name);
-    teal_print("\t space order     : %d", eqns->space_order);
-    teal_print("\t convective flux : %s", eqns->convective.name);
-    teal_print("\t viscous flux    : %s", eqns->viscous.name);
-    teal_print("\t limiter         : %s", eqns->limiter.name);
+    for (int i = 0; i < 5; i++) {
+        teal_print("\t %-15s : %s", label[i], value[i]);
+    }
     teal_print("\t Boundary conditions");
     for (int i = 0; i < eqns->boundary.num; i++) {
-        teal_print("\t\t %-20s : %s", eqns->mesh->entities.name[num_inner + i],
+        teal_print("\t\t %-20s : %s", entity_name(eqns, eqns->mesh->entities.num_inner + i),
                    eqns->boundary.name[i]);
     }
     teal_print("\t Properties");
@@ -263,25 +277,88 @@ typedef struct {
     const double *eps2; // parameter pointer it was built from
     int *bc;
     double *ctx;
+    // bulk-velocity forcing (teal_equations_set_forcing): entity, {U, tau}, and what the
+    // device handle currently has (-1: nothing applied yet)
+    int forcing_entity;
+    double forcing[2];
+    int applied_source;
+    double applied_forcing[2];
 } Device;
 
 static Device *device_state(const Equations *eqns)
 {
-    return (Device *)eqns->device;
+    Equations *mut = (Equations *)eqns;
+    if (!mut->device) {
+        Device *dev = teal_calloc(1, sizeof(Device));
+        dev->precision = TEAL_FP64;
+        dev->applied_source = -1;
+        mut->device = dev;
+    }
+    return (Device *)mut->device;
+}
+
+static void device_drop(Device *dev)
+{
+    if (dev->gpu) {
+        teal_gpu_destroy(dev->gpu);
+        dev->gpu = 0;
+    }
+    dev->applied_source = -1;
 }
 
 void teal_equations_set_precision(Equations *eqns, int precision)
 {
     Device *dev = device_state(eqns);
-    if (dev && dev->gpu && dev->precision != precision) {
-        teal_gpu_destroy(dev->gpu);
-        dev->gpu = 0;
-    }
-    if (!dev) {
-        dev = teal_calloc(1, sizeof(Device));
-        eqns->device = dev;
+    if (dev->precision != precision) {
+        device_drop(dev);
     }
     dev->precision = precision;
+}
+
+// ------------------------------------------------------------------ forcing source
+// The bulk-velocity forcing of run/periodic_hill/run.c:8-21 as library callbacks: the device
+// path recognises this pair and runs it as TEAL_SOURCE_FORCING.  The host bodies follow the
+// reference driver (its Prepare keeps the average in a file-scope record).
+static struct {
+    double velocity, relaxation, average;
+} forcing_host = {0, 1, 0};
+
+void teal_forcing_prepare(const Equations *eqns, const void *primitive)
+{
+    Device *dev = device_state(eqns);
+    double avg[5];
+    equations_average(eqns, entity_name(eqns, dev->forcing_entity), primitive, avg);
+    forcing_host.velocity = dev->forcing[0];
+    forcing_host.relaxation = dev->forcing[1];
+    forcing_host.average = avg[1];
+}
+
+void teal_forcing_source(void *source, const double *property, Vector center, double time,
+                         const void *context)
+{
+    (void)property, (void)center, (void)time;
+    const double *p = context;  // the cell's primitive state {rho, u, v, w, p}
+    double *s = source;
+    const double s1 = (p[0] * (forcing_host.velocity - forcing_host.average)) /
+                      forcing_host.relaxation;
+    s[0] = s[2] = s[3] = 0;
+    s[1] = s1;
+    s[4] = p[1] * s1;
+}
+
+void teal_equations_set_forcing(Equations *eqns, const char *entity, double velocity,
+                                double relaxation)
+{
+    assert(eqns && entity && relaxation != 0);
+    int e = find_name(eqns->mesh->entities.name, 0, eqns->mesh->entities.num_inner, entity);
+    if (e < 0) {
+        teal_error("invalid entity (%s)", entity);
+    }
+    Device *dev = device_state(eqns);
+    dev->forcing_entity = e;
+    dev->forcing[0] = velocity;
+    dev->forcing[1] = relaxation;
+    equations_set_source(eqns, teal_forcing_source, teal_forcing_prepare);
 }
 
 // build the device configuration from the plugin callbacks; teal_error when a callback has
@@ -306,8 +383,10 @@ static void device_config(const Equations *eqns, TealGpuConfig *cfg, int *bc, do
     if (ts < 0) {
         teal_error("time step callback of \"%s\" has no device implementation", eqns->name);
     }
-    if (eqns->source.compute || eqns->source.prepare) {
-        teal_error("source terms are host callbacks with no device implementation");
+    if ((eqns->source.compute || eqns->source.prepare) &&
+        (eqns->source.compute != teal_forcing_source || eqns->source.prepare != teal_forcing_prepare)) {
+        teal_error("source term callbacks have no device implementation (the device source is "
+                   "teal_equations_set_forcing)");
     }
     cfg->space_order = eqns->space_order;
     cfg->flux = flux;
@@ -345,13 +424,7 @@ static void device_config(const Equations *eqns, TealGpuConfig *cfg, int *bc, do
 // the configured device mirror of eqns (created on first use)
 TealGpu *teal_equations_device(const Equations *eqns)
 {
-    Equations *mut = (Equations *)eqns;
     Device *dev = device_state(eqns);
-    if (!dev) {
-        dev = teal_calloc(1, sizeof(Device));
-        dev->precision = TEAL_FP64;
-        mut->device = dev;
-    }
     if (!dev->gpu) {
         dev->gpu = teal_gpu_create(eqns->mesh, dev->precision);
         if (!dev->gpu) {
@@ -385,6 +458,14 @@ TealGpu *teal_equations_device(const Equations *eqns)
     }
     teal_free(bc);
     teal_free(ctx);
+    const int source = eqns->source.compute ? TEAL_SOURCE_FORCING : TEAL_SOURCE_NONE;
+    if (source != dev->applied_source || memcmp(dev->forcing, dev->applied_forcing, sizeof(dev->forcing))) {
+        if (teal_gpu_set_source(dev->gpu, source, dev->forcing_entity, dev->forcing)) {
+            teal_error("%s", teal_gpu_last_error());
+        }
+        dev->applied_source = source;
+        memcpy(dev->applied_forcing, dev->forcing, sizeof(dev->forcing));
+    }
     return dev->gpu;
 }
 
@@ -410,6 +491,9 @@ void equations_boundary(const Equations *eqns, void *primitive, double time)
 {
     assert(eqns && primitive);
     (void)time;
+    if (eqns->boundary.num == 0) {  // no boundary entities: no ghost states to compute
+        return;
+    }
     check(teal_gpu_boundary(teal_equations_device(eqns), primitive));
 }
 
@@ -441,99 +525,59 @@ void equations_residual(const Equations *eqns, const void *derivative, void *res
     check(teal_gpu_residual(teal_equations_device(eqns), derivative, residual));
 }
 
-// ------------------------------------------------------------------ diagnostics (host)
-// volume-weighted entity average of a host array (src/equations/average.c:8-41)
-void equations_average(const Equations *eqns, const char *entity, const void *primitive_,
-                       void *average_)
+// ------------------------------------------------------------------ diagnostics
+// volume-weighted entity average of a host array (src/equations/average.c:8-41), on the device
+void equations_average(const Equations *eqns, const char *entity, const void *primitive,
+                       void *average)
 {
-    assert(eqns && entity);
-    const int stride = eqns->primitive.stride;
-    const double *primitive = primitive_;
-    double *average = average_;
-    memset(average, 0, stride * sizeof(double));
-    for (int i = 0; i < eqns->mesh->entities.num_inner; i++) {
-        if (!strcmp(eqns->mesh->entities.name[i], entity)) {
-            for (int j = eqns->mesh->entities.cell_off[i]; j < eqns->mesh->entities.cell_off[i + 1];
-                 j++) {
-                for (int k = 0; k < stride; k++) {
-                    average[k] += eqns->mesh->cells.volume[j] * primitive[((size_t)j * stride) + k];
-                }
-            }
-            for (int k = 0; k < stride; k++) {
-                average[k] /= eqns->mesh->cells.sum_volume;
-            }
-            return;
-        }
+    assert(eqns && entity && average);
+    int e = find_name(eqns->mesh->entities.name, 0, eqns->mesh->entities.num_inner, entity);
+    if (e < 0) {
+        teal_error("invalid entity (%s)", entity);
     }
-    teal_error("invalid entity (%s)", entity);
+    check(teal_gpu_average(teal_equations_device(eqns), e, primitive, average));
 }
 
-// L2 error against the reference solution callback (src/equations/norm.c:8-38)
-void equations_norm(const Equations *eqns, void *norm_, double time)
+// L2 error against the reference solution (src/equations/norm.c:8-38): the reference callback
+// fills eqns->reference on the host (it is user code), the sum runs on the device
+void equations_norm(const Equations *eqns, void *norm, double time)
 {
-    assert(eqns && norm_);
-    const int stride = eqns->primitive.stride;
-    const double *primitive = eqns->primitive.data;
-    double *reference = eqns->reference.data;
-    double *norm = norm_;
-    memset(norm, 0, stride * sizeof(double));
-    if (!eqns->reference.func.compute) {
+    assert(eqns && norm);
+    Compute *exact = eqns->reference.func.compute;
+    if (!exact) {
         teal_error("no reference solution registered");
     }
+    const int stride = eqns->primitive.stride;
+    const double *prim = eqns->primitive.data;
+    double *ref = eqns->reference.data;
     for (int i = 0; i < eqns->mesh->cells.num_inner; i++) {
-        double *ref = &reference[(size_t)i * stride];
-        const double *p = &primitive[(size_t)i * stride];
-        eqns->reference.func.compute(ref, eqns->properties.data, eqns->mesh->cells.center[i], time,
-                                     p);
-        for (int j = 0; j < stride; j++) {
-            norm[j] += eqns->mesh->cells.volume[i] * sq(ref[j] - p[j]);
-        }
+        exact(&ref[(size_t)i * stride], eqns->properties.data, eqns->mesh->cells.center[i], time,
+              &prim[(size_t)i * stride]);
     }
-    for (int j = 0; j < stride; j++) {
-        norm[j] = sqrt(norm[j] / eqns->mesh->cells.sum_volume);
-    }
-}
-
-void equations_write(const Equations *eqns, const char *name, double time, int index)
-{
-    (void)eqns, (void)time;
-    static int warned = 0;
-    if (!warned) {
-        teal_print("\t (VTKHDF output is not provided by this library: %s_%05d skipped)", name, index);
-        warned = 1;
-    }
-}
-
-void equations_read(const Equations *eqns, const char *fname, double *time, int *index)
-{
-    (void)eqns, (void)time, (void)index;
-    teal_error("restart input is not provided by this library (%s)", fname);
+    check(teal_gpu_norm(teal_equations_device(eqns), prim, ref, norm));
 }
 
 void equations_destroy(Equations *eqns)
 {
     assert(eqns);
-    Device *dev = device_state(eqns);
-    if (dev) {
-        if (dev->gpu) {
-            teal_gpu_destroy(dev->gpu);
-        }
+    if (eqns->device) {
+        Device *dev = device_state(eqns);
+        device_drop(dev);
         teal_free(dev->bc);
         teal_free(dev->ctx);
         teal_free(dev);
     }
-    teal_free(eqns->boundary.name);
-    teal_free(eqns->boundary.context);
-    teal_free(eqns->boundary.compute);
-    teal_free(eqns->boundary.condition);
-    EquationsVariables *vars[] = {&eqns->primitive, &eqns->conserved, &eqns->reference};
-    for (int i = 0; i < 3; i++) {
-        teal_free(vars[i]->name);
-        teal_free(vars[i]->dimension);
-        teal_free(vars[i]->data);
+    EquationsVariables *sets[] = {&eqns->primitive, &eqns->conserved, &eqns->reference};
+    void *owned[] = {eqns->boundary.name, eqns->boundary.context, eqns->boundary.compute,
+                     eqns->boundary.condition, eqns->properties.name, eqns->properties.data,
+                     eqns->limiter.parameter};
+    for (size_t i = 0; i < sizeof(owned) / sizeof(owned[0]); i++) {
+        teal_free(owned[i]);
     }
-    teal_free(eqns->properties.name);
-    teal_free(eqns->properties.data);
-    teal_free(eqns->limiter.parameter);
+    for (int i = 0; i < 3; i++) {
+        teal_free(sets[i]->name);
+        teal_free(sets[i]->dimension);
+        teal_free(sets[i]->data);
+    }
     teal_free(eqns);
 }

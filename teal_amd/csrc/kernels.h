@@ -569,15 +569,21 @@ struct StageArgs {
     double *partials;     // MODE 2: [num_blocks][5] residual partial sums
     double *dtpart;       // MODE 2: [num_blocks] minima of the local time step
     int keep_u;           // MODE 2: next U0 = U (mixed) instead of prim2cons(P_new) (FP64)
+    // bulk-velocity forcing source, or null: {entity average of the primitive state (5),
+    // target velocity U, relaxation time tau}; the cell's source is s1 = rho (U - <u>) / tau
+    // in the x momentum and u s1 in the energy (the source and Prepare hook of
+    // run/periodic_hill/run.c:8-21, evaluated on the state the residual is taken of)
+    const double *forcing = nullptr;
 };
 
 // Stage epilogue of inner cell c (active) given its accumulated face sums: finalize
 // R = -R/V (derivative.c:253-258); MODE 0 writes R; MODE 1 writes P_new = cons2prim(U0 +
 // (R alpha) step) (advance.c:103-108); MODE 2 also produces the next step's U0, the dt minimum
 // and the residual partial sums of this workgroup (block-wide: all threads must call it).
+// Pin is the state the residual is taken of (the source term's context); it may equal Pout.
 template <typename S, int MODE, int NT = BLOCK>
 __device__ inline void stage_epilogue(const Topo &t, const Geom<S> &g, const Phys &ph, const Acc<S> &acc,
-                                      int c, bool active, int blk, S *__restrict__ Pout,
+                                      int c, bool active, int blk, const S *Pin, S *Pout,
                                       const StageArgs &a)
 {
     const int ni = t.ni;
@@ -592,6 +598,13 @@ __device__ inline void stage_epilogue(const Topo &t, const Geom<S> &g, const Phy
         else {  // FP32 path: one reciprocal, FP64 products
             const double iv = 1.0 / vol;
             for (int v = 0; v < 5; v++) r[v] = -acc.get(v) * iv;
+        }
+        if (a.forcing) {  // R = (-R/V) + source, every component (derivative.c:260-265)
+            const double rho = (double)Pin[(long)c * 5];
+            const double u = (double)Pin[((long)c * 5) + 1];
+            const double s1 = (rho * (a.forcing[5] - a.forcing[1])) / a.forcing[6];
+            const double src[5] = {0.0, s1, 0.0, 0.0, u * s1};
+            for (int v = 0; v < 5; v++) r[v] = r[v] + src[v];
         }
         if (MODE == 0) {
             for (int v = 0; v < 5; v++) a.R[((long)v * ni) + c] = r[v];
@@ -709,9 +722,11 @@ __device__ inline void accumulate(const Topo &t, const int *__restrict__ slots, 
 }
 
 // Accumulate from the face flux array F written by k_face.
+// Pout is also the state the residual is taken of (the face fluxes are already in F): the
+// source term reads a cell's row before its update overwrites it, so Pout is not restrict.
 template <typename S, int VISC, int CONV, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_accum(Topo t, Geom<S> g, Phys ph, const S *__restrict__ F,
-                                                 S *__restrict__ Pout, StageArgs a)
+                                                 S *Pout, StageArgs a)
 {
     using R = FaceRec<S, VISC, CONV>;
     const int blk = xcd_block();
@@ -725,7 +740,7 @@ __global__ __launch_bounds__(BLOCK) void k_accum(Topo t, Geom<S> g, Phys ph, con
             for (int v = 0; v < 5; v++) x[v] = rec[v];
         });
     }
-    stage_epilogue<S, MODE>(t, g, ph, acc, c, active, blk, Pout, a);
+    stage_epilogue<S, MODE>(t, g, ph, acc, c, active, blk, Pout, Pout, a);
 }
 
 // Per-block face lists of the fused kernel: the workgroup of device cells
@@ -795,7 +810,7 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux_block(Topo t, BlockTopo b
             for (int v = 0; v < 5; v++) x[v] = lds[(((part ? R::CONV_OFF : 0) + v) * mf) + j];
         });
     }
-    stage_epilogue<S, MODE, FLUX_THREADS>(t, g, ph, acc, c, active, blk, Pout, a);
+    stage_epilogue<S, MODE, FLUX_THREADS>(t, g, ph, acc, c, active, blk, P, Pout, a);
 }
 
 // ---------------------------------------------------------------- time step + U0

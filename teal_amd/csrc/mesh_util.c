@@ -283,15 +283,31 @@ void mesh_destroy(Mesh *mesh)
     teal_free(mesh);
 }
 
+// The ghost block of neighbour entry e receives the send list of the entry whose send tag is
+// e's receive tag (src/equations/exchange.c:19-33, src/mesh/create.c:409-419): src[ghost] =
+// its source cell; the other entries of src are left as they are.
+void teal_mesh_periodic_sources(const Mesh *mesh, int *src)
+{
+    const MeshNeighbors *nb = &mesh->neighbors;
+    for (int e = 0; e < nb->num; e++) {
+        int s = -1;
+        for (int k = 0; k < nb->num; k++) {
+            if (nb->tag[k][1] == nb->tag[e][0]) {
+                s = k;
+            }
+        }
+        if (s < 0) {
+            continue;
+        }
+        for (int j = nb->recv_off[e]; j < nb->recv_off[e + 1]; j++) {
+            src[j] = nb->send.idx[nb->send.off[s] + (j - nb->recv_off[e])];
+        }
+    }
+}
+
 Mesh *mesh_read(const char *fname)
 {
     teal_error("mesh_read is not provided by this library (%s)", fname);
-}
-
-void mesh_write(const Mesh *mesh, const char *name)
-{
-    (void)mesh;
-    teal_error("mesh_write is not provided by this library (%s)", name);
 }
 
 // ------------------------------------------------------------------ tetrahedral mesh
