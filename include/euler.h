@@ -36,7 +36,8 @@ Equations *euler_create(const Mesh *mesh); // src/euler.h:29
 EulerPrimitive euler_riemann(EulerPrimitive left, EulerPrimitive right, double gamma,
                              double loc);  // src/euler.h:32
 
-// Airfoil polar post-processing (src/euler.h:35): not provided by this library (teal_error).
+// Airfoil polar post-processing (src/euler.h:35, src/euler/polar.c:22-83): force sums on the
+// device, lift and drag printed, <prefix>_polar_<alpha>.h5 written when the prefix is set.
 void euler_polar(const Simulation *sim, const char *entity, EulerPrimitive reference,
                  double length, double time);
 

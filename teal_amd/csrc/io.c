@@ -30,6 +30,7 @@
 
 #include "equations.h"
 #include "mesh.h"
+#include "plugins.h"
 #include "teal.h"
 
 // ------------------------------------------------------------------ HDF5 helpers
@@ -488,4 +489,21 @@ void equations_read(const Equations *eqns, const char *fname, double *time, int 
     teal_free(buf);
     teal_free(state);
     teal_free(center);
+}
+
+// ------------------------------------------------------------------ polar file
+void teal_io_write_polar(const char *fname, double alpha_deg, const Vector *center,
+                         const double *pressure_c, int num, double lift, double drag)
+{
+    h5_load();
+    hid_t file = h5.H5Fcreate(fname, FILE_TRUNCATE, H5P_DEFAULT, H5P_DEFAULT);
+    if (file < 0) {
+        teal_error("could not create file (%s)", fname);
+    }
+    put(file, "alpha", &alpha_deg, 1, 1, h5.t_double);
+    put(file, "center", center, num, 3, h5.t_double);
+    put(file, "pressure_c", pressure_c, num, 1, h5.t_double);
+    put(file, "lift_c", &lift, 1, 1, h5.t_double);
+    put(file, "drag_c", &drag, 1, 1, h5.t_double);
+    h5.H5Fclose(file);
 }

@@ -10,7 +10,9 @@
 #include "navierstokes.h"
 #include "physics.h"
 #include "plugins.h"
+#include "simulation.h"
 #include "teal.h"
+#include "teal_gpu.h"
 
 using namespace fv;
 
@@ -246,10 +248,45 @@ extern "C" Equations *navierstokes_create(const Mesh *mesh)
     return eqns;
 }
 
-// Airfoil polar post-processing is not provided by this library (src/euler/polar.c:22-83).
+// src/euler/polar.c:22-83: the boundary pass and the face sums on the device (teal_gpu_polar),
+// lift and drag from the force coefficients, and with an output prefix the per-face pressure
+// coefficients in <prefix>_polar_<alpha in degrees>.h5
 extern "C" void euler_polar(const Simulation *sim, const char *entity, EulerPrimitive reference,
                             double length, double time)
 {
-    (void)sim, (void)reference, (void)length, (void)time;
-    teal_error("euler_polar is not provided by this library (%s)", entity);
+    (void)time;  // the device boundary conditions do not depend on time
+    const Equations *eqns = sim->eqns;
+    const Mesh *mesh = eqns->mesh;
+    int idx = -1;
+    for (int i = 0; i < mesh->entities.off_boundary && idx < 0; i++) {
+        if (!strcmp(mesh->entities.name[i], entity)) {
+            idx = i;
+        }
+    }
+    if (idx < 0) {
+        teal_error("invalid entity (%s)", entity);
+    }
+    const int f0 = mesh->entities.face_off[idx];
+    const int num = mesh->entities.face_off[idx + 1] - f0;
+    const double ref[5] = {reference.density, reference.velocity.x, reference.velocity.y,
+                           reference.velocity.z, reference.pressure};
+    double coef[2];
+    double *pressure_c = (double *)teal_calloc(num + 1, sizeof(double));
+    if (teal_gpu_polar(teal_equations_device(eqns), idx, (double *)eqns->primitive.data, ref,
+                       length, coef, pressure_c)) {
+        teal_error("%s", teal_gpu_last_error());
+    }
+    const double alpha = atan2(reference.velocity.y, reference.velocity.x);
+    const double lift = (cos(alpha) * coef[1]) - (sin(alpha) * coef[0]);
+    const double drag = (cos(alpha) * coef[0]) + (sin(alpha) * coef[1]);
+    teal_print("Airfoil polar");
+    teal_print("\t lift coefficient : %g", lift);
+    teal_print("\t drag coefficient : %g", drag);
+    if (sim->prefix[0]) {
+        const double alpha_deg = alpha * 180 / acos(-1.0);
+        char fname[512];
+        snprintf(fname, sizeof(fname), "%s_polar_%g.h5", sim->prefix, alpha_deg);
+        teal_io_write_polar(fname, alpha_deg, &mesh->faces.center[f0], pressure_c, num, lift, drag);
+    }
+    teal_free(pressure_c);
 }
