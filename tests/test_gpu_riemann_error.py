@@ -17,9 +17,12 @@ def _states(arrays):
     bad[:, 0] = 1.0
     bad[:, 1] = np.where(x < 0.5, -5.0, 5.0)
     bad[:, 4] = 1e-2
+    # a smooth subsonic wave; it must move and carry a pressure variation, or its residual is
+    # exactly zero (a stationary contact with uniform pressure is a steady state)
     clean = np.zeros((nc, 5))
     clean[:, 0] = 1.0 + 0.1 * np.sin(2 * np.pi * x)
-    clean[:, 4] = 1.0
+    clean[:, 1] = 0.1
+    clean[:, 4] = 1.0 + 0.05 * np.cos(2 * np.pi * x)
     return bad, clean
 
 

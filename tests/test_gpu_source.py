@@ -96,10 +96,21 @@ def test_forcing_lserk_steps():
     gpu.close()
 
 
+# FP64: the default solver at CFL 20, bit for bit with serial reductions.  Mixed (FP32 matvec
+# residuals, FP64 Newton residual): both paths stop Newton at the same relative tolerance, so
+# they differ by about that tolerance times the step's change (0.1 x ~5e-2 at CFL 20 with the
+# default solver); the mixed case runs the tight solver at CFL 5 and is held to 1e-3 of the
+# largest |U| per conserved variable, the bound of the mixed implicit tests.
+IMPLICIT = {teal_amd.FP64: (20.0, (0.1, 0.3, 30)), teal_amd.MIXED: (5.0, (1e-3, 1e-2, 10))}
+
+
 @pytest.mark.parametrize("precision", [teal_amd.FP64, teal_amd.MIXED], ids=["fp64", "mixed"])
 def test_forcing_implicit(precision):
     """Implicit Euler with the forcing source (the matvec's residuals carry the source too)."""
+    from test_gpu_implicit import MIXED_RTOL, conserved, relative_error
+
     print(f"forcing implicit precision={precision}", flush=True)
+    courant, solver = IMPLICIT[precision]
     om, oe, a, gpu = setup(CHANNEL, precision, serial=True)
     nc, ni = a["num_cells"], a["num_inner"]
     P = channel_state(a)
@@ -108,16 +119,18 @@ def test_forcing_implicit(precision):
     gpu.upload(P)
     t_o, res_o, t_g = C.c_double(0.0), (C.c_double * 5)(), 0.0
     for _ in range(3):
-        s_o = oracle().oracle_implicit_euler(oe, C.byref(t_o), res_o, 1e300, 20.0, 0.1, 0.3, 30)
-        s_g, t_g, res_g = gpu.implicit_euler(t_g, 20.0, 0.1, 0.3, 30)
+        s_o = oracle().oracle_implicit_euler(oe, C.byref(t_o), res_o, 1e300, courant, *solver)
+        s_g, t_g, res_g = gpu.implicit_euler(t_g, courant, *solver)
         if precision == teal_amd.FP64:
             assert s_g == s_o
             np.testing.assert_allclose(res_g, np.array(res_o[:]), rtol=1e-12)
     Po, Pg = po.reshape(-1, 5)[:ni], gpu.download()[:ni]
     if precision == teal_amd.FP64:
         np.testing.assert_array_equal(Pg, Po)
-    else:  # FP32 matvec: the Newton solution agrees to the solver tolerances
-        assert np.abs(Pg - Po).max() <= 1e-4 * np.abs(Po).max()
+    else:
+        err = relative_error(conserved(Pg, CHANNEL.gamma), conserved(Po, CHANNEL.gamma))
+        print(f"  state error / max |U| per variable: {err}", flush=True)
+        assert (err <= MIXED_RTOL).all(), err
     gpu.close()
 
 
