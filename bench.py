@@ -509,12 +509,9 @@ def batch_workload(args, seed=2026):
         "runs": runs}), flush=True)
 
 
-def channel_workload(args, n=64, seed=5):
-    """Navier-Stokes implicit Euler with Jacobian-free Newton-GMRES (NewtonKrylov {0.1, 0.3,
-    30}) in a channel [0,2] x [-1,1] x [0,1]: periodic in x and z, no-slip walls in y with the
-    node levels stretched geometrically (ratio 1.05) toward both walls, parabolic profile plus
-    1% seeded noise, Re = 100 on the half height, M = 0.2, HLLC + central viscous, BJ.
-    `args.steps` implicit steps at CFL `args.courant`; FP64 and mixed."""
+def channel_mesh(n):
+    """The 64^3-style channel [0,2] x [-1,1] x [0,1]: periodic in x and z, node levels in y
+    stretched geometrically (ratio 1.05) toward both walls."""
     lib = teal_amd.lib()
     mesh = lib.mesh_create(teal_amd.Vector(0, -1, 0), teal_amd.Vector(2, 1, 1),
                            teal_amd.Triple(n, n, n), teal_amd.Triple(1, 0, 1))
@@ -525,6 +522,22 @@ def channel_workload(args, n=64, seed=5):
     coord = _abi.mesh_arrays(mesh)["coord"]
     coord[:, 1] = levels[np.rint((coord[:, 1] + 1) / (2.0 / n)).astype(int)]
     lib.mesh_generate(mesh)
+    return mesh
+
+
+def channel_workload(args, n=64, seed=5):
+    """Navier-Stokes implicit Euler with Jacobian-free Newton-GMRES (NewtonKrylov {0.1, 0.3,
+    30}) in the stretched channel of channel_mesh: no-slip (adiabatic) walls in y, parabolic
+    profile plus 1% seeded noise, Re = 100 on the half height, M = 0.2, HLLC + central viscous,
+    unlimited second order.  Runs each path until the largest residual component falls below
+    1e-8 of its first value (the reference's termination test, src/simulation/run.c:80-82,
+    taken relative), or `args.steps` steps, or `args.budget` seconds.
+
+    No forcing: the reference's only Navier-Stokes wall is adiabatic
+    (src/navierstokes/boundary.c:6-26), so a bulk-velocity forcing source (periodic_hill)
+    feeds the gas heat it can never lose and the energy residual never falls; the steady state
+    reached here is the decay of the initial flow to rest."""
+    mesh = channel_mesh(n)
     a = _abi.mesh_arrays(mesh)
     nc, ni = a["num_cells"], a["num_inner"]
     y = a["center"][:nc, 1]
@@ -536,42 +549,55 @@ def channel_workload(args, n=64, seed=5):
     out = {}
     for name, prec in (("fp64", teal_amd.FP64), ("mixed", teal_amd.MIXED)):
         g = teal_amd.Gpu(mesh, prec)
-        g.configure(space_order=2, flux="hllc", viscous=1, limiter="minmod", navierstokes=1,
+        g.configure(space_order=2, flux="hllc", viscous=1, limiter=None, navierstokes=1,
                     gamma=GAMMA, mu=0.01, prandtl=PR,
                     bcs={"bottom": ("wall", None), "top": ("wall", None)})
         g.upload(P0)
         g.implicit_euler(0.0, args.courant, 0.1, 0.3, 30)  # warm-up step (allocations)
         g.upload(P0)
-        t, newton, krylov, res0 = 0.0, 0, 0, None
+        t, newton, krylov, res0, hist = 0.0, 0, 0, None, []
         t0 = time.perf_counter()
-        for i in range(args.steps):
+        steps = 0
+        while steps < args.steps:
             _, t, res = g.implicit_euler(t, args.courant, 0.1, 0.3, 30)
+            steps += 1
             st = g.krylov_stats()
             newton += st["newton"]
             krylov += st["krylov"]
-            res0 = res if res0 is None else res0
-            print(f"#   {name} step {i}: {st['newton']} Newton / {st['krylov']} GMRES iterations, "
-                  f"residual {max(res):.3g}", file=sys.stderr, flush=True)
+            r = float(max(res))
+            res0 = r if res0 is None else res0
+            elapsed = time.perf_counter() - t0
+            if steps % 250 == 1 or r <= 1e-8 * res0:
+                hist.append((steps, t, r / res0))
+                print(f"#   {name} step {steps}: t {t:.4g}, residual {r:.3e} (drop {r / res0:.2e}), "
+                      f"{newton} Newton / {krylov} GMRES iterations, {elapsed:.1f} s",
+                      file=sys.stderr, flush=True)
+            if r <= 1e-8 * res0 or elapsed > args.budget:
+                break
         secs = time.perf_counter() - t0
         g.close()
-        out[name] = {"ms_per_step": 1e3 * secs / args.steps, "newton_iterations": newton,
+        converged = r <= 1e-8 * res0
+        out[name] = {"converged": converged, "steps": steps, "seconds": secs,
+                     "time_to_solution_s": secs if converged else None,
+                     "ms_per_step": 1e3 * secs / steps, "newton_iterations": newton,
                      "gmres_iterations": krylov, "matvecs_per_s": krylov / secs,
-                     "residual_first": float(max(res0)), "residual_last": float(max(res)),
-                     "sim_time": t}
-        print(f"# channel {name}: {1e3 * secs / args.steps:.2f} ms/implicit step, {newton} Newton / "
-              f"{krylov} GMRES iterations in {args.steps} steps, {krylov / secs:.0f} matvecs/s, "
-              f"residual {max(res0):.3g} -> {max(res):.3g}", file=sys.stderr, flush=True)
+                     "residual_first": res0, "residual_last": r, "drop": r / res0,
+                     "sim_time": t, "history": hist}
+        print(f"# channel {name}: {'converged' if converged else 'stopped'} after {steps} steps, "
+              f"{secs:.1f} s, drop {r / res0:.2e}, {newton} Newton / {krylov} GMRES iterations, "
+              f"{krylov / secs:.0f} matvecs/s", file=sys.stderr, flush=True)
     mx = out["mixed"]
     print(json.dumps({
-        "metric": "implicit Euler (Newton-GMRES) matvecs/s, NS channel 64^3",
-        "value": mx["matvecs_per_s"], "unit": "matvecs/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": 1, "ms_per_step": mx["ms_per_step"], "higher_is_better": True,
+        "metric": "implicit Euler (Newton-GMRES) time to a 1e-8 residual drop, NS channel 64^3",
+        "value": mx["time_to_solution_s"], "unit": "s", "n_gpus": 1, "steps": mx["steps"],
+        "warmup": 1, "ms_per_step": mx["ms_per_step"], "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 matvec residuals + f64 Krylov vectors / Gram-Schmidt (mixed)",
         "data": "synthetic",
-        "config": {"workload": f"NS channel {n}^3, walls stretched 1.05, Re 100, M 0.2, implicit "
-                               f"Euler CFL {args.courant}, NewtonKrylov (0.1, 0.3, 30)",
-                   "cells": ni},
+        "config": {"workload": f"NS channel {n}^3, walls stretched 1.05, Re 100, M 0.2, unforced "
+                               f"decay to rest, implicit Euler CFL {args.courant}, NewtonKrylov "
+                               "(0.1, 0.3, 30), unlimited second order",
+                   "cells": ni, "termination": "max residual <= 1e-8 x first"},
         "runs": out}), flush=True)
 
 
@@ -579,7 +605,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
-                    help="timed steps (default 1000; tet and batch 500; channel 8 implicit steps)")
+                    help="timed steps (default 1000; tet and batch 500; channel: at most 30000 "
+                         "implicit steps toward steady state)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--accuracy-steps", type=int, default=None)
@@ -601,14 +628,16 @@ def main():
                          "NS implicit Euler (Newton-GMRES) in a 64^3 stretched-wall channel; "
                          "batch: 256 Euler shock tubes of 16^3..64^3 cells, batched vs one at "
                          "a time")
-    ap.add_argument("--courant", type=float, default=50.0, help="channel workload CFL")
+    ap.add_argument("--courant", type=float, default=200.0, help="channel workload CFL")
+    ap.add_argument("--budget", type=float, default=240.0,
+                    help="channel workload: wall-clock seconds per path before it stops")
     ap.add_argument("--channel-n", type=int, default=64, help="channel workload cells per side")
     args = ap.parse_args()
     if args.cpu_worker:
         cpu_worker(args.cpu_worker)
         return
     if args.steps is None:  # the step counts of the configurations
-        args.steps = {"batch": 500, "tet": 500, "channel": 8}.get(args.workload, 1000)
+        args.steps = {"batch": 500, "tet": 500, "channel": 30000}.get(args.workload, 1000)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this library drives one GPU per process; runs on several GPUs are "
               "not part of it", file=sys.stderr)

@@ -85,7 +85,8 @@ typedef struct {
 // Cartesian hex mesh with optional per-axis periodicity (src/mesh.h:75, create.c:509-544).
 Mesh *mesh_create(Vector min_coord, Vector max_coord, Triple num_cells, Triple periodic);
 
-// Gmsh input (src/mesh.h:78): not provided by this library; reports an error through teal_error.
+// Gmsh 4.1 input, ASCII or binary, with periodic links (src/mesh.h:78, src/mesh/read.c:901-918,
+// src/mesh/gmsh.c:135-904), read on one process.  Errors end the program through teal_error.
 Mesh *mesh_read(const char *fname);
 
 // Modify all node coordinates (src/mesh.h:81, src/mesh/modify.c:5-11).

@@ -65,6 +65,7 @@ def lib():
         _sig(l, "mesh_create", VP, Vector, Vector, Triple, Triple)
         _sig(l, "mesh_create_tets", VP, C.c_int, D, C.c_ulonglong)
         _sig(l, "mesh_generate", None, VP)
+        _sig(l, "mesh_read", VP, C.c_char_p)
         _sig(l, "mesh_validate", None, VP)
         _sig(l, "mesh_destroy", None, VP)
         _sig(l, "teal_gpu_create", VP, VP, C.c_int)

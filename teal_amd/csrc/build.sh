@@ -32,7 +32,7 @@ build_hip diagnostics.hip -ffp-contract=off & pids+=($!)
 # is loaded at run time (io.c)
 H5DIR="${TEAL_HDF5:-/opt/conda}"
 gcc "${CFLAGS[@]}" -I"$H5DIR/include" -c "$HERE/io.c" -o "$OBJ/io.o" & pids+=($!)
-for src in core.c sync.c mesh_create.c mesh_generate.c mesh_util.c mesh_union.c equations.c simulation.c; do
+for src in core.c sync.c mesh_create.c mesh_generate.c mesh_read.c mesh_util.c mesh_union.c equations.c simulation.c; do
     if [ -f "$HERE/$src" ]; then
         gcc "${CFLAGS[@]}" -c "$HERE/$src" -o "$OBJ/${src%.*}.o" & pids+=($!)
     fi

@@ -305,11 +305,6 @@ void teal_mesh_periodic_sources(const Mesh *mesh, int *src)
     }
 }
 
-Mesh *mesh_read(const char *fname)
-{
-    teal_error("mesh_read is not provided by this library (%s)", fname);
-}
-
 // ------------------------------------------------------------------ tetrahedral mesh
 static uint64_t splitmix64(uint64_t *state)
 {
