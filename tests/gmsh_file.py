@@ -121,41 +121,38 @@ class Msh:
             f.write(bytes(out))
 
 
-def from_mesh(a, node_tags=None, periodic_links=()):
-    """A Msh holding an ungenerated library mesh `a` (the arrays of _abi.mesh_arrays before
-    mesh_generate) with the same node, cell and entity order: one node block, one volume
-    block (physical "domain"), one surface block per boundary or periodic entity (physical
-    tags in the mesh's entity order, so the reader's sort keeps it).  `periodic_links` are
-    (slave entity name, master entity name, translation) triples."""
+ELEMENT_TYPE = {(3, 4): 4, (3, 8): 5, (3, 6): 6, (3, 5): 7, (2, 3): 2, (2, 4): 3}
+
+
+def from_cells(coord, cells, num_inner_entities, names, off, node_tags=None, periodic_links=()):
+    """A Msh of cells given as node lists (indices into coord), entity e holding cells
+    off[e] .. off[e+1] (the first `num_inner_entities` entities volumes, the rest faces): one
+    node block, and per entity one element block per element type, in the order the types
+    first appear (the reader keeps that block order within an entity).  Physical tags follow
+    the entity order.  `periodic_links` are (slave entity name, master entity name,
+    translation) triples."""
     m = Msh()
-    nn = a["num_nodes"]
+    coord = np.asarray(coord, dtype=float)
+    nn = len(coord)
     tags = np.arange(1, nn + 1) * 3 + 7 if node_tags is None else np.asarray(node_tags)
-    m.node_blocks.append((3, 1, tags, a["coord"]))
-    names = a["entity_names"]
-    off = a["entity_cell_off"]
-    noff, nidx = a["cell_node_off"], a["cell_node_idx"]
+    m.node_blocks.append((3, 1, tags, coord))
     el = 1
     for e, name in enumerate(names):
-        dim = 3 if e < a["entities_num_inner"] else 2
+        dim = 3 if e < num_inner_entities else 2
         m.physicals.append((dim, e + 1, name))
         ent_tag = 100 + e
         (m.volumes if dim == 3 else m.surfaces).append((ent_tag, [e + 1]))
-        cells = range(int(off[e]), int(off[e + 1]))
-        by_len = {}
-        for c in cells:
-            nodes = nidx[noff[c]:noff[c + 1]]
-            by_len.setdefault(len(nodes), []).append(tags[nodes])
-        if len(by_len) != 1:
-            raise ValueError("one element type per entity")
-        (k, rows), = by_len.items()
-        etype = {(3, 4): 4, (3, 8): 5, (3, 6): 6, (3, 5): 7, (2, 3): 2, (2, 4): 3}[(dim, k)]
-        m.blocks.append((dim, ent_tag, etype, np.arange(el, el + len(rows)), np.array(rows)))
-        el += len(rows)
-    coord = a["coord"]
+        by_type = {}
+        for c in range(int(off[e]), int(off[e + 1])):
+            nodes = list(cells[c])
+            by_type.setdefault(ELEMENT_TYPE[(dim, len(nodes))], []).append(tags[nodes])
+        for etype, rows in by_type.items():
+            m.blocks.append((dim, ent_tag, etype, np.arange(el, el + len(rows)), np.array(rows)))
+            el += len(rows)
     for slave, master, t in periodic_links:
         s, mm = names.index(slave), names.index(master)
-        snodes = np.unique(nidx[noff[off[s]]:noff[off[s + 1]]])
-        mnodes = np.unique(nidx[noff[off[mm]]:noff[off[mm + 1]]])
+        snodes = np.unique([v for c in range(off[s], off[s + 1]) for v in cells[c]])
+        mnodes = np.unique([v for c in range(off[mm], off[mm + 1]) for v in cells[c]])
         pairs = []
         for n in snodes:  # master node at slave node - t
             d = np.abs(coord[mnodes] - (coord[n] - np.asarray(t))).max(axis=1)
@@ -166,3 +163,27 @@ def from_mesh(a, node_tags=None, periodic_links=()):
         affine[:3, 3] = t
         m.links.append((2, 100 + s, 100 + mm, affine.reshape(-1), np.array(pairs)))
     return m
+
+
+def file_order(cells, num_inner_entities, off):
+    """The cell order mesh_read gives a file written by from_cells: per entity, grouped by
+    element type in first-appearance order."""
+    order = []
+    for e in range(len(off) - 1):
+        dim = 3 if e < num_inner_entities else 2
+        groups = {}
+        for c in range(int(off[e]), int(off[e + 1])):
+            groups.setdefault(ELEMENT_TYPE[(dim, len(cells[c]))], []).append(c)
+        for g in groups.values():
+            order.extend(g)
+    return order
+
+
+def from_mesh(a, node_tags=None, periodic_links=()):
+    """from_cells of an ungenerated library mesh (the arrays of _abi.mesh_arrays before
+    mesh_generate): the same node, cell and entity order when each entity has one element
+    type."""
+    noff, nidx = a["cell_node_off"], a["cell_node_idx"]
+    cells = [nidx[noff[c]:noff[c + 1]] for c in range(a["num_cells"])]
+    return from_cells(a["coord"], cells, a["entities_num_inner"], a["entity_names"],
+                      a["entity_cell_off"], node_tags, periodic_links)

@@ -218,3 +218,32 @@ def test_read_mesh_runs_on_device(tmp_path, precision):
     np.testing.assert_array_equal(gpu.download()[:ni], po.reshape(nc, 5)[:ni])
     gpu.close()
     lib.mesh_destroy(mesh)
+
+
+def test_mixed_hex_prism_mesh_against_oracle(tmp_path):
+    """A box with columns split into prisms (two element types in the volume, quads and
+    triangles on the z sides) written with one element block per type: mesh_read keeps the
+    blocks' order, and after mesh_generate equals the oracle's mesh_generate on the same cells
+    in that order, bit for bit."""
+    from mixed_mesh import mixed_box, oracle_mixed_mesh
+    from oracle_lib import oracle
+    from gmsh_file import file_order, from_cells
+
+    lib = teal_amd.lib()
+    coord, cells, ni, names, off = mixed_box((4, 3, 2), (0.5, 0.4, 0.3), {(1, 0), (2, 1), (0, 2)})
+    path = tmp_path / "mixed.msh"
+    from_cells(coord, cells, 1, names, off).write(path)
+    order = file_order(cells, 1, off)
+    om = oracle_mixed_mesh(coord, [cells[c] for c in order], ni, names, off)
+    oracle().oracle_mesh_generate(om)
+    got = read(path)
+    lib.mesh_generate(got)
+    want = _abi.mesh_arrays(om)
+    a = _abi.mesh_arrays(got)
+    assert a["num_inner"] == ni and len(set(len(c) for c in cells[:ni])) == 2
+    for key, w in want.items():
+        if isinstance(w, np.ndarray):
+            np.testing.assert_array_equal(a[key], w, err_msg=key)
+        else:
+            assert a[key] == w, key
+    lib.mesh_destroy(got)
