@@ -88,6 +88,15 @@ int teal_gpu_limiter(TealGpu *gpu, const double *primitive, double *gradient); /
 int teal_gpu_derivative(TealGpu *gpu, double *primitive, double *derivative); // derivative.c:269
 double teal_gpu_timestep(TealGpu *gpu, const double *primitive); // src/equations/timestep.c:7
 int teal_gpu_residual(TealGpu *gpu, const double *derivative, double *residual); // residual.c:8
+// No counterpart in the reference.  teal_gpu_residual for callers that cannot allocate (code
+// under HIP graph capture, serving loops): it allocates nothing and does not synchronize.
+// `derivative` is a DEVICE array in the reference layout (inner cells x 5 doubles), `residual`
+// a DEVICE array of 5 doubles, written on `stream` (a hipStream_t; null: the handle's stream)
+// with the same values as teal_gpu_residual.  Block partial sums live in a fixed 2.5 MiB
+// device array: meshes of more than 16,777,216 inner cells and batch handles are refused.
+// Calls that overlap on one GPU race on that array.
+int teal_gpu_residual_noalloc(TealGpu *gpu, const double *derivative, double *residual,
+                              void *stream);
 
 // Resident state: the device copy of the primitive variables is authoritative between
 // teal_gpu_upload and teal_gpu_download.

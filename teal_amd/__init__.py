@@ -78,6 +78,7 @@ def lib():
         _sig(l, "teal_gpu_derivative", C.c_int, VP, DP, DP)
         _sig(l, "teal_gpu_timestep", D, VP, DP)
         _sig(l, "teal_gpu_residual", C.c_int, VP, DP, DP)
+        _sig(l, "teal_gpu_residual_noalloc", C.c_int, VP, VP, VP, VP)
         _sig(l, "teal_gpu_upload", C.c_int, VP, DP)
         _sig(l, "teal_gpu_download", C.c_int, VP, DP)
         _sig(l, "teal_gpu_euler", D, VP, DP, DP, D, D)

@@ -113,15 +113,10 @@ constexpr int STEP_END_SUMS = 16;
 static_assert(STEP_END_THREADS % 5 == 0 && (STEP_END_SUMS & (STEP_END_SUMS - 1)) == 0,
               "k_step_end layout");
 
-__global__ __launch_bounds__(STEP_END_THREADS) void k_step_end(Scalars *scalars, const double *partials,
-                                                               const int *blk_off,
-                                                               const double *volume, int advance_time)
+// The sums of k_step_end: red[v] (v < 5, block shared memory of STEP_END_THREADS doubles) =
+// the sum of variable v over partials[e0 .. e1) (rows of 5) in k_step_end's fixed order.
+__device__ inline void sum_partials(const double *partials, long e0, long e1, double *red)
 {
-    __shared__ double red[STEP_END_THREADS];
-    Scalars *sc = &scalars[blockIdx.x];
-    const double sum_volume = volume[blockIdx.x];
-    const long e0 = (long)blk_off[blockIdx.x] * 5;
-    const long e1 = (long)blk_off[blockIdx.x + 1] * 5;
     const int t = threadIdx.x;
     const int lanes = STEP_END_THREADS / 5;
     double acc[STEP_END_SUMS] = {};
@@ -135,8 +130,7 @@ __global__ __launch_bounds__(STEP_END_THREADS) void k_step_end(Scalars *scalars,
     for (int h = STEP_END_SUMS / 2; h > 0; h >>= 1) {
         for (int u = 0; u < h; u++) acc[u] += acc[u + h];
     }
-    double s = acc[0];
-    red[t] = s;
+    red[t] = acc[0];
     __syncthreads();
     for (int h = lanes / 2; h > 0; h >>= 1) {
         if (t < h * 5) {
@@ -144,7 +138,17 @@ __global__ __launch_bounds__(STEP_END_THREADS) void k_step_end(Scalars *scalars,
         }
         __syncthreads();
     }
-    if (t == 0) {
+}
+
+__global__ __launch_bounds__(STEP_END_THREADS) void k_step_end(Scalars *scalars, const double *partials,
+                                                               const int *blk_off,
+                                                               const double *volume, int advance_time)
+{
+    __shared__ double red[STEP_END_THREADS];
+    Scalars *sc = &scalars[blockIdx.x];
+    const double sum_volume = volume[blockIdx.x];
+    sum_partials(partials, (long)blk_off[blockIdx.x] * 5, (long)blk_off[blockIdx.x + 1] * 5, red);
+    if (threadIdx.x == 0) {
         for (int v = 0; v < 5; v++) sc->residual[v] = sqrt(red[v] / sum_volume);
         if (advance_time) {
             sc->time += sc->step;
@@ -152,16 +156,30 @@ __global__ __launch_bounds__(STEP_END_THREADS) void k_step_end(Scalars *scalars,
     }
 }
 
-// residual partials of a stored derivative [5][ni]
-__global__ void k_residual_partials(const double *__restrict__ R, const double *__restrict__ vol64,
-                                    int ni, double *partials)
+// out[v] = sqrt(sum V R_v^2 / sum V) from `nb` block partials: k_step_end's sums for one
+// instance, written to a caller's array.  One workgroup.
+__global__ __launch_bounds__(STEP_END_THREADS) void k_residual_out(const double *partials, int nb,
+                                                                   const double *volume, double *out)
+{
+    __shared__ double red[STEP_END_THREADS];
+    sum_partials(partials, 0, (long)nb * 5, red);
+    if (threadIdx.x < 5) {
+        out[threadIdx.x] = sqrt(red[threadIdx.x] / volume[0]);
+    }
+}
+
+// residual partials of a stored derivative: device rows [5][ni] (perm null), or host rows in
+// the reference layout [rows][5], device row c reading host row perm[c]
+__global__ void k_residual_partials(const double *__restrict__ R, const int *__restrict__ perm,
+                                    const double *__restrict__ vol64, int ni, double *partials)
 {
     __shared__ double red[BLOCK / 64][5];
     int c = (blockIdx.x * BLOCK) + threadIdx.x;
     double s[5] = {0, 0, 0, 0, 0};
     if (c < ni) {
+        const long h = perm ? perm[c] : -1;
         for (int v = 0; v < 5; v++) {
-            double r = R[((long)v * ni) + c];
+            double r = perm ? ((h >= 0) ? R[(h * 5) + v] : 0.0) : R[((long)v * ni) + c];
             s[v] = vol64[c] * (r * r);
         }
     }
@@ -1328,7 +1346,7 @@ int teal_residual_norm(TealGpu *g, const double *d_derivative, double *residual)
 {
     const double *vol64 = (g->precision == TEAL_FP64) ? g->d64.d_vol64 : g->d32.d_vol64;
     hipLaunchKernelGGL(k_residual_partials, dim3(g->nblocks), dim3(BLOCK), 0, g->stream, d_derivative,
-                       vol64, g->ni, g->d_partials);
+                       (const int *)nullptr, vol64, g->ni, g->d_partials);
     return read_residuals(g, residual);
 }
 
@@ -1343,8 +1361,67 @@ extern "C" int teal_gpu_residual(TealGpu *g, const double *derivative, double *r
     Kern<double>::to_device(g->stream, g->d_stage, g->d_R, g->ni, 5, 5, 5, g->d_cell_perm, 1);
     const double *vol64 = (g->precision == TEAL_FP64) ? g->d64.d_vol64 : g->d32.d_vol64;
     hipLaunchKernelGGL(k_residual_partials, dim3(g->nblocks), dim3(BLOCK), 0, g->stream, g->d_R,
-                       vol64, g->ni, g->d_partials);
+                       (const int *)nullptr, vol64, g->ni, g->d_partials);
     return read_residuals(g, residual);
+}
+
+// ------------------------------------------------------------------ residual, no allocation
+// teal_gpu_residual_noalloc keeps its block partial sums in this fixed array rather than in
+// memory it allocates.  In host code the name `scratch` is not a device address: the address
+// of the current device's copy comes from hipGetSymbolAddress, once per device, and only
+// that address is passed to kernels.  Every call on a device shares its copy; each call
+// overwrites the partials it reads (nothing has to start at zero), and calls that overlap on
+// one device race on it.
+constexpr int NOALLOC_MAX_BLOCKS = 65536;  // inner cells <= 65536 * BLOCK (16,777,216)
+static __device__ __align__(256) double scratch[(size_t)NOALLOC_MAX_BLOCKS * 5];  // 2.5 MiB
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static double *scratch_dev[NOALLOC_MAX_DEVICES];
+
+static int scratch_of_current_device(double **out)
+{
+    int dev = -1;
+    HIPCALL(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+        teal_gpu_set_error("residual_noalloc: device %d beyond the %d supported", dev,
+                           NOALLOC_MAX_DEVICES);
+        return -1;
+    }
+    if (!scratch_dev[dev]) {
+        void *p = nullptr;
+        HIPCALL(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        scratch_dev[dev] = (double *)p;
+    }
+    *out = scratch_dev[dev];
+    return 0;
+}
+
+extern "C" int teal_gpu_residual_noalloc(TealGpu *g, const double *derivative, double *residual,
+                                         void *stream)
+{
+    if (check_ready(g)) return -1;
+    if (g->ninst > 1) {
+        teal_gpu_set_error("residual_noalloc: not available on a batch handle");
+        return -1;
+    }
+    if (!derivative || !residual) {
+        teal_gpu_set_error("residual_noalloc: null device array");
+        return -1;
+    }
+    if (g->nblocks > NOALLOC_MAX_BLOCKS) {
+        teal_gpu_set_error("residual_noalloc: %d inner cells exceed the limit of %ld", g->ni,
+                           (long)NOALLOC_MAX_BLOCKS * BLOCK);
+        return -1;
+    }
+    double *part = nullptr;
+    if (scratch_of_current_device(&part)) return -1;
+    hipStream_t s = stream ? (hipStream_t)stream : g->stream;
+    const double *vol64 = (g->precision == TEAL_FP64) ? g->d64.d_vol64 : g->d32.d_vol64;
+    hipLaunchKernelGGL(k_residual_partials, dim3(g->nblocks), dim3(BLOCK), 0, s, derivative,
+                       (const int *)g->d_cell_perm, vol64, g->ni, part);
+    hipLaunchKernelGGL(k_residual_out, dim3(1), dim3(STEP_END_THREADS), 0, s, (const double *)part,
+                       g->nblocks, (const double *)g->d_inst_volume, residual);
+    HIPCALL(hipGetLastError());
+    return 0;
 }
 
 // ------------------------------------------------------------------ resident state
