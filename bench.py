@@ -89,6 +89,13 @@ def measured_traffic(precision, path=os.path.join(ROOT, "profiles", "hbm_traffic
             "source": os.path.relpath(path, ROOT)}
 
 
+def picked_shape(ms):
+    """The autotuner's pick from its timings: the fastest stage shape / gradient workgroup."""
+    nst = len(teal_amd.AUTOTUNE_STAGE_SHAPES)
+    names = teal_amd.AUTOTUNE_SHAPES
+    return names[int(np.nanargmin(ms[:nst]))] + "/" + names[nst + int(np.nanargmin(ms[nst:]))]
+
+
 def make_gpu(mesh, precision):
     g = teal_amd.Gpu(mesh, precision)
     g.configure(space_order=2, flux="hllc", viscous=1, limiter="minmod", navierstokes=1,
@@ -105,7 +112,7 @@ def timed_run(g, P0, steps, warmup, time_kernels, tune_steps=10):
     g.upload(P0)
     if warmup:
         teal_amd.check(lib.teal_gpu_run_lserk(g.h, warmup, C.byref(t), CFL, 2, 3), "warmup")
-    shapes = ("fused", "fused+graph", "split", "split+graph")
+    shapes = teal_amd.AUTOTUNE_SHAPES
     ms = g.autotune(max(2, tune_steps), CFL) if tune_steps else None
     g.upload(P0)
     t = C.c_double(0.0)
@@ -115,7 +122,7 @@ def timed_run(g, P0, steps, warmup, time_kernels, tune_steps=10):
     teal_amd.check(rc, "run_lserk")
     out = {"seconds": t1 - t0, "sim_time": t.value}
     if ms is not None:
-        out["shape"] = shapes[int(np.nanargmin(ms))]
+        out["shape"] = picked_shape(ms)
         out["autotune_ms_per_step"] = dict(zip(shapes, [None if np.isnan(x) else x for x in ms.tolist()]))
     if time_kernels:
         ksteps = min(steps, 100)
@@ -452,8 +459,8 @@ def batch_workload(args, seed=2026):
         b.configure(**cfg)
         b.upload_batch(prims)
         ms = b.autotune(max(2, args.tune_steps), CFL)
-        shapes = ("fused", "fused+graph", "split", "split+graph")
-        best = shapes[int(np.nanargmin(ms))]
+        shapes = teal_amd.AUTOTUNE_SHAPES
+        best = picked_shape(ms)
         times = np.zeros(args.batch)
         if args.warmup:
             b.run_lserk_batch(args.warmup, times, CFL)
@@ -488,7 +495,7 @@ def batch_workload(args, seed=2026):
     ua = _abi.mesh_arrays(u)
     model = byte_model(ua["num_inner"], ua["num_cells"], ua["num_faces"], mixed=False, viscous=False)
     lib.mesh_destroy(u)
-    fused = fp["layout"]["fused"] == 1 and fp["shape"].startswith("fused")
+    fused = fp["layout"]["fused"] == 1
     kern = "stage" if fused else "accum"
     achieved = model[kern] / (fp["stage_ms"] * 1e-3) / 1e9 if fp["stage_ms"] > 0 else None
     print(json.dumps({

@@ -92,8 +92,8 @@ int teal_gpu_residual(TealGpu *gpu, const double *derivative, double *residual);
 // under HIP graph capture, serving loops): it allocates nothing and does not synchronize.
 // `derivative` is a DEVICE array in the reference layout (inner cells x 5 doubles), `residual`
 // a DEVICE array of 5 doubles, written on `stream` (a hipStream_t; null: the handle's stream)
-// with the same values as teal_gpu_residual.  Block partial sums live in a fixed 2.5 MiB
-// device array: meshes of more than 16,777,216 inner cells and batch handles are refused.
+// with the same values as teal_gpu_residual.  Partial sums live in a fixed 10 MiB device
+// array: meshes of more than 16,777,216 inner cells and batch handles are refused.
 // Calls that overlap on one GPU race on that array.
 int teal_gpu_residual_noalloc(TealGpu *gpu, const double *derivative, double *residual,
                               void *stream);
@@ -183,10 +183,18 @@ int teal_gpu_run_lserk_batch(TealGpu *gpu, int steps, double *time, double *resi
 int teal_gpu_set_graphs(TealGpu *gpu, int enable);
 
 // Launch-shape autotuner (no counterpart in the reference): times `steps` lserk steps of the
-// current state in each shape the handle supports -- ms[0] fused stage kernel, ms[1] fused
-// with graph replay, ms[2] face-flux + accumulation kernels, ms[3] those with graph replay
-// (per-step milliseconds, NaN where the shape is unavailable) -- keeps the fastest shape and
-// restores the state, time and residual it started from.
+// current state in each shape the handle supports and keeps the fastest, then restores the
+// state, time and residual it started from.  ms[] receives per-step milliseconds (NaN where a
+// shape is unavailable), TEAL_AUTOTUNE_SHAPES values:
+//   ms[0], ms[1]  fused stage kernel on blocks of 256 device rows; eager, graph replay
+//   ms[2], ms[3]  fused stage kernel on blocks of 128 rows (its own face lists); eager, graph
+//   ms[4], ms[5]  face-flux + accumulation kernels; eager, graph replay
+//   ms[6..8]      the chosen stage shape with gradient-kernel workgroups of 256, 128, 64
+// Every shape gives the same results, bit for bit in FP64.
+enum { TEAL_AUTOTUNE_STAGE_SHAPES = 6, TEAL_AUTOTUNE_SHAPES = 9 };
+// Sets a stage shape (the index of ms[0..5] above) and a gradient workgroup size (64, 128 or
+// 256) directly; an error if the shape does not fit the device.
+int teal_gpu_set_shape(TealGpu *gpu, int shape, int grad_block);
 int teal_gpu_autotune(TealGpu *gpu, int steps, double courant, int time_order, int num_stages,
                       double *ms);
 
@@ -203,8 +211,11 @@ double teal_gpu_timing_ms(TealGpu *gpu, int kernel, int *launches);
 // that makes every instance whole 256-row blocks), cells, faces, inner faces, offset of the
 // periodic faces, maximum faces per cell, whether residual passes run the fused stage kernel,
 // and whether it reads the centre form of the face geometry (offsets and corrections
-// recomputed from face and cell centres).
-int teal_gpu_info(TealGpu *gpu, long *counts /* [8]: ni, nc, nf, nfi, nfb, W, fused, centre */);
+// recomputed from face and cell centres), and how the implicit solver forms a GMRES column.
+// counts[11]: ni, nc, nf, nfi, nfb, W, fused, centre form, rows per fused block (0: split),
+// gradient workgroup size, vector elements per thread of the one-launch GMRES column (0:
+// separate kernels per Gram-Schmidt step, or no implicit step yet)
+int teal_gpu_info(TealGpu *gpu, long *counts);
 
 // Precision of the device mirror behind an Equations (no counterpart in the reference,
 // whose path is FP64 only): TEAL_FP64 (the default) or TEAL_MIXED.  Takes effect at the

@@ -14,6 +14,23 @@ void teal_gpu_set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2
 
 struct TealGpu;
 
+// face lists of the fused stage kernel for one block layout (fv::BlockTopo): blocks of `rows`
+// device rows, each with the faces touching it, their geometry in list order
+template <typename S>
+struct BlockSet {
+    int rows = 0;  // device rows per block (0: not built)
+    int *d_boff = nullptr, *d_bslot = nullptr;
+    int2 *d_bflr = nullptr;
+    S *d_bpairs = nullptr;  // face records in block-list order as pairs (FP32 [5][nj][2])
+    S *d_barea = nullptr, *d_bbasis = nullptr, *d_boffl = nullptr, *d_boffr = nullptr,
+      *d_bcorr = nullptr;  // face geometry in block-list order (stride num_block_faces)
+    S *d_bfcen = nullptr;  // centre form: face centres in block-list order
+    int *d_bcrow = nullptr;  // centre form: right-cell centre row per list entry
+    bool centre_form = false;
+    int num_block_faces = 0;
+    int max_block_faces = 0;
+};
+
 // device arrays of one precision (see kernels.h for the layout)
 template <typename S>
 struct DevData {
@@ -30,38 +47,33 @@ struct DevData {
     S *d_GP = nullptr;   // [num_inner][20]
     S *d_F = nullptr;    // [num_faces][10] (split path only)
     S *d_eps2 = nullptr;
-    // per-block face lists of the fused kernel (fv::BlockTopo)
-    int *d_boff = nullptr, *d_bslot = nullptr;
-    int2 *d_bflr = nullptr;
-    S *d_bpairs = nullptr;  // FP32: face records in block-list order as pairs ([5][nj][2])
-    S *d_barea = nullptr, *d_bbasis = nullptr, *d_boffl = nullptr, *d_boffr = nullptr,
-      *d_bcorr = nullptr;  // face geometry in block-list order (stride num_block_faces)
-    S *d_bfcen = nullptr;  // centre form: face centres in block-list order
+    // the two block layouts of the fused kernel: lay[0] blocks of fv::BLOCK rows, lay[1] of
+    // fv::SMALL_BLOCK; `cur` is the one the residual passes use (autotuned)
+    BlockSet<S> lay[2];
+    int cur = 0;
     S *d_ccen = nullptr;   // centre form: cell centres [num_cells][3], device rows
-    int *d_bcrow = nullptr;  // centre form: right-cell centre row per list entry
-    bool centre_form = false;
-    int num_block_faces = 0;
-    int max_block_faces = 0;
     int max_inc = fv::W;  // most incidences of any inner cell
     int fused = 0;  // 1: residual passes use k_flux_block (its face lists fit in LDS and the
                     // environment does not set TEAL_GPU_SPLIT=1)
 
     int build(const Mesh *mesh, TealGpu *gpu);
-    int build_blocks(const std::vector<int> &inc, int ni, const std::vector<int2> &flr,
-                     const std::vector<double> &area, const std::vector<double> &basis,
-                     const std::vector<double> &offl, const std::vector<double> &offr,
-                     const std::vector<double> &corr, const std::vector<double> &fcen,
-                     const std::vector<int> &crow, const std::vector<double> &ccen);
+    int build_blocks(BlockSet<S> &b, int rows, const std::vector<int> &inc, int ni,
+                     const std::vector<int2> &flr, const std::vector<double> &area,
+                     const std::vector<double> &basis, const std::vector<double> &offl,
+                     const std::vector<double> &offr, const std::vector<double> &corr,
+                     const std::vector<double> &fcen, const std::vector<int> &crow,
+                     const std::vector<double> &ccen);
     void release();
     fv::Geom<S> geom() const;
     fv::BlockTopo block_topo() const;
     fv::Geom<S> block_geom() const;  // cell geometry + face geometry in block-list order
+    int block_rows() const { return lay[cur].rows; }
     // dynamic LDS of the fused stage kernel: the face records of the largest block list
     // (values per face as FaceRec: 10 for FP64 with viscous and convective fluxes, else 5)
     size_t block_lds(int visc = 1, int conv = 1) const
     {
         const int nv = (sizeof(S) == 8 && visc && conv) ? 10 : 5;
-        return sizeof(S) * nv * (size_t)max_block_faces;
+        return sizeof(S) * nv * (size_t)lay[cur].max_block_faces;
     }
 };
 
@@ -69,15 +81,15 @@ struct DevData {
 template <typename S>
 struct Kern {
     static void boundary(hipStream_t s, int num, int nf, const DevData<S> &d, double gamma);
-    static void grad_limit(hipStream_t s, int lim, int mode, fv::Topo t, fv::Geom<S> g,
+    static void grad_limit(hipStream_t s, int lim, int mode, int gb, fv::Topo t, fv::Geom<S> g,
                            const S *P, S *GP);
     static int face(hipStream_t s, int flux, int visc, int order, fv::Topo t, fv::Geom<S> g,
                     fv::Phys ph, const S *P, const S *GP, S *F, int *error);
     static void accum(hipStream_t s, int visc, int conv, int mode, fv::Topo t, fv::Geom<S> g,
                       fv::Phys ph, const S *F, S *P, fv::StageArgs a);
-    static int flux_block(hipStream_t s, int flux, int visc, int order, int mode, size_t lds,
-                          fv::Topo t, fv::BlockTopo bt, fv::Geom<S> g, fv::Phys ph, const S *P,
-                          const S *GP, S *Pout, fv::StageArgs a);
+    static int flux_block(hipStream_t s, int flux, int visc, int order, int mode, int bs,
+                          size_t lds, fv::Topo t, fv::BlockTopo bt, fv::Geom<S> g, fv::Phys ph,
+                          const S *P, const S *GP, S *Pout, fv::StageArgs a);
     static void timestep_u0(hipStream_t s, fv::Topo t, fv::Geom<S> g, fv::Phys ph,
                             const S *P, double *U0, double *dtpart, int write_u0);
     // host layout (reference numbering) <-> device layout (device numbering);
@@ -118,9 +130,11 @@ struct TealGpu {
     // host rows of instance i in the union: inner, boundary-ghost and periodic-ghost bases and
     // counts, inst_rows[6 * i + 0..5]
     std::vector<int> inst_rows;
-    int *d_blk_inst = nullptr;
-    int *d_inst_blk_off = nullptr;
-    int *d_all_blk = nullptr;  // {0, nblocks}: every block as one range
+    // partial sums (dt minima, residual sums) are kept per group of 64 device rows, whatever
+    // the block layout: instance i owns groups [inst_grp_off[i], inst_grp_off[i+1])
+    int *d_grp_inst = nullptr;     // instance of each group
+    int *d_inst_grp_off = nullptr;
+    int *d_all_grp = nullptr;      // {0, ngroups}: every group as one range
     double *d_inst_volume = nullptr;
     std::vector<int> periodic_src;
     std::vector<int> cell_perm, cell_inv, face_perm, face_inv;  // device <-> reference numbering
@@ -135,14 +149,19 @@ struct TealGpu {
     int configured = 0;
     int u0_valid = 0;  // d_U0 holds the conserved state of the current step start
     int dt_valid = 0;  // the scalars hold the dt minimum of that state
-    int nblocks = 0;
+    int nblocks = 0;   // blocks of fv::BLOCK rows
+    int ngroups = 0;   // groups of 64 rows
+    int grad_block = fv::BLOCK;  // threads per workgroup of the gradient kernel (autotuned)
     hipStream_t stream = nullptr;
     hipEvent_t ev_wait = nullptr;  // host waits on the stream by polling this event
     double *d_stage = nullptr;
     double *d_U0 = nullptr;
     double *d_R = nullptr;
-    double *d_partials = nullptr;
-    double *d_dtpart = nullptr;  // [nblocks] per-block time-step minima
+    double *d_partials = nullptr;  // [ngroups][5] residual partial sums
+    double *d_dtpart = nullptr;    // [ngroups] time-step minima
+    double *d_chunk = nullptr;     // chunk results of the per-step reductions
+    unsigned *d_tickets = nullptr; // [ninst] arrivals of a reduction's workgroups, 0 between launches
+    int max_inst_groups = 0;       // groups of the largest instance
     fv::Scalars *d_scalars = nullptr;
     fv::Scalars *h_scalars = nullptr;
     int timing = 0;
@@ -181,6 +200,9 @@ int teal_residual_pass(TealGpu *g, DevData<S> &d, int mode, double alpha, int la
 template <typename S>
 int teal_timestep_pass(TealGpu *g, DevData<S> &d, int write_u0, double *out);
 void teal_krylov_release(TealGpu *g);
+// elements per thread of the fused GMRES column kernel in use, 0: separate kernels (or no
+// implicit step yet)
+int teal_krylov_column_ept(const TealGpu *g);
 void teal_diagnostics_release(TealGpu *g);
 // average of the forcing source's entity over the state P into d_forcing (no-op without a
 // source): the Prepare step of every residual evaluation

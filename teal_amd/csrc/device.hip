@@ -63,37 +63,101 @@ __global__ void k_init_scalars(Scalars *scalars, const double *times, double tim
     sc->error = 0;
 }
 
-// dt minimum over the per-block minima (src/equations/timestep.c:22-29: a min, so the order
-// does not matter); begin != 0 also sets step0 = courant * dt and step = min(step0, max_step)
-// for the step about to run (src/simulation/advance.c:92-93).  Workgroup i reduces the blocks
-// [blk_off[i], blk_off[i + 1]) of instance i into scalars[i].
-constexpr int DT_THREADS = 1024;
+// The per-step reductions read one partial per group of 64 rows (a 128^3 mesh has 32,768).
+// A single workgroup would spend tens of microseconds on the load latency, so each instance's
+// groups are cut into fixed chunks, one workgroup per chunk (grid: chunks x instances).  Each
+// chunk result goes to `chunk`, the workgroup that finishes an instance's chunks last (a
+// ticket per instance) combines the chunk results in chunk order and writes the instance's
+// scalars, then resets the ticket for the next launch.  The order of every sum is fixed by the
+// chunk and group numbering alone, never by which workgroup finishes first.
+//
+// Instance i with groups [g0, g1) keeps its chunk results in rows g0 / chunk_size + i onwards
+// of `chunk`: those rows never overlap the next instance's, so `chunk` needs
+// ngroups / chunk_size + ninst rows.
+__device__ inline long chunk_row0(int g0, int inst, int chunk_size)
+{
+    return (long)(g0 / chunk_size) + inst;
+}
 
-__global__ __launch_bounds__(DT_THREADS) void k_dt_min(Scalars *scalars, const double *dtpart,
-                                                      const int *blk_off, int begin, double courant,
-                                                      double max_step)
+// workgroups of a chunked reduction over `n` groups
+static inline unsigned chunks_of(int n, int chunk_size)
+{
+    return (unsigned)std::max(1, (n + chunk_size - 1) / chunk_size);
+}
+
+// true in every thread of the workgroup that arrives last of `n` at `ticket` (then with the
+// other workgroups' writes visible and the ticket reset to 0)
+__device__ inline bool last_workgroup(unsigned *ticket, unsigned n)
+{
+    __shared__ int last;
+    __threadfence();  // this workgroup's chunk result visible device-wide before its ticket
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        last = (atomicAdd(ticket, 1u) == n - 1);
+        if (last) {
+            atomicExch(ticket, 0u);
+        }
+    }
+    __syncthreads();
+    if (last) {
+        __threadfence();  // and the others' results visible to this workgroup
+    }
+    return last;
+}
+
+// dt minimum over the minima of the groups of 64 rows (src/equations/timestep.c:22-29: a min,
+// so the order does not matter); begin != 0 also sets step0 = courant * dt and step =
+// min(step0, max_step) for the step about to run (src/simulation/advance.c:92-93).  Instance
+// blockIdx.y owns the groups [blk_off[y], blk_off[y + 1]); its result goes to scalars[y].
+constexpr int DT_THREADS = 1024;
+constexpr int DT_CHUNK = 4 * DT_THREADS;  // groups per workgroup: four loads per thread
+
+__device__ inline double block_min(double dt)
 {
     __shared__ double red[DT_THREADS / 64];
-    Scalars *sc = &scalars[blockIdx.x];
-    const int b0 = blk_off[blockIdx.x];
-    const int b1 = blk_off[blockIdx.x + 1];
-    double dt = __longlong_as_double(0x7ff0000000000000LL);
-    double d[4] = {dt, dt, dt, dt};  // four loads in flight per thread
-    int b = b0 + threadIdx.x;
-    for (; b + (3 * DT_THREADS) < b1; b += 4 * DT_THREADS) {
-        for (int u = 0; u < 4; u++) d[u] = fmin(d[u], dtpart[b + (u * DT_THREADS)]);
-    }
-    for (; b < b1; b += DT_THREADS) {
-        d[0] = fmin(d[0], dtpart[b]);
-    }
-    dt = fmin(fmin(d[0], d[1]), fmin(d[2], d[3]));
     for (int off = 32; off > 0; off >>= 1) dt = fmin(dt, __shfl_xor(dt, off));
     if ((threadIdx.x & 63) == 0) {
         red[threadIdx.x >> 6] = dt;
     }
     __syncthreads();
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) dt = fmin(dt, red[w]);
+    return dt;
+}
+
+__global__ __launch_bounds__(DT_THREADS) void k_dt_min(Scalars *scalars, const double *dtpart,
+                                                      const int *blk_off, double *chunk,
+                                                      unsigned *tickets, int begin, double courant,
+                                                      double max_step)
+{
+    const int inst = blockIdx.y;
+    const int g0 = blk_off[inst];
+    const int g1 = blk_off[inst + 1];
+    const int nch = std::max(1, (g1 - g0 + DT_CHUNK - 1) / DT_CHUNK);
+    if ((int)blockIdx.x >= nch) return;
+    const double inf = __longlong_as_double(0x7ff0000000000000LL);
+    double d[4] = {inf, inf, inf, inf};  // four loads in flight per thread
+    const int c0 = g0 + (blockIdx.x * DT_CHUNK);
+    const int c1 = std::min(c0 + DT_CHUNK, g1);
+    for (int u = 0; u < 4; u++) {
+        int b = c0 + (u * DT_THREADS) + threadIdx.x;
+        if (b < c1) {
+            d[u] = dtpart[b];
+        }
+    }
+    double dt = block_min(fmin(fmin(d[0], d[1]), fmin(d[2], d[3])));
+    if (nch > 1) {
+        double *ch = chunk + chunk_row0(g0, inst, DT_CHUNK);
+        if (threadIdx.x == 0) {
+            ch[blockIdx.x] = dt;
+        }
+        if (!last_workgroup(&tickets[inst], nch)) return;
+        dt = inf;
+        for (int x = threadIdx.x; x < nch; x += DT_THREADS) dt = fmin(dt, ch[x]);
+        __syncthreads();  // block_min's shared array is reused
+        dt = block_min(dt);
+    }
     if (threadIdx.x == 0) {
-        for (int w = 1; w < DT_THREADS / 64; w++) dt = fmin(dt, red[w]);
+        Scalars *sc = &scalars[inst];
         sc->dtmin_bits = (unsigned long long)__double_as_longlong(dt);
         if (begin) {
             sc->step0 = courant * dt;
@@ -102,19 +166,20 @@ __global__ __launch_bounds__(DT_THREADS) void k_dt_min(Scalars *scalars, const d
     }
 }
 
-// residual = sqrt(sum V R^2 / sum V) from per-block partials [nblocks][5] in a fixed order
-// (src/equations/residual.c:8-33), then time += step (advance.c:111).  Workgroup i handles
-// the blocks [blk_off[i], blk_off[i + 1]) of instance i, of total volume volume[i].  Thread t
-// of the STEP_END_THREADS = 5 x 128 sums variable t % 5 of blocks t / 5, t / 5 + 128, ...
-// (coalesced reads) into STEP_END_SUMS running sums taken round-robin (that many loads in
-// flight), adds those pairwise, then a fixed tree over the 128 partial sums.
+// residual = sqrt(sum V R^2 / sum V) from the partials of the groups of 64 rows [ngroups][5] in
+// a fixed order (src/equations/residual.c:8-33), then time += step (advance.c:111).
+// Within a chunk of STEP_END_CHUNK groups, thread t of the STEP_END_THREADS = 5 x 128 sums
+// variable t % 5 of groups t / 5, t / 5 + 128, ... (coalesced reads) into STEP_END_SUMS running
+// sums taken round-robin (that many loads in flight), adds those pairwise, then a fixed tree
+// over the 128 partial sums.  The chunk sums are then summed the same way, in chunk order.
 constexpr int STEP_END_THREADS = 640;
 constexpr int STEP_END_SUMS = 16;
+constexpr int STEP_END_CHUNK = STEP_END_SUMS * (STEP_END_THREADS / 5);  // one load round
 static_assert(STEP_END_THREADS % 5 == 0 && (STEP_END_SUMS & (STEP_END_SUMS - 1)) == 0,
               "k_step_end layout");
 
-// The sums of k_step_end: red[v] (v < 5, block shared memory of STEP_END_THREADS doubles) =
-// the sum of variable v over partials[e0 .. e1) (rows of 5) in k_step_end's fixed order.
+// red[v] (v < 5, block shared memory of STEP_END_THREADS doubles) = the sum of variable v over
+// partials[e0 .. e1) (rows of 5) in the fixed order above
 __device__ inline void sum_partials(const double *partials, long e0, long e1, double *red)
 {
     const int t = threadIdx.x;
@@ -130,6 +195,7 @@ __device__ inline void sum_partials(const double *partials, long e0, long e1, do
     for (int h = STEP_END_SUMS / 2; h > 0; h >>= 1) {
         for (int u = 0; u < h; u++) acc[u] += acc[u + h];
     }
+    __syncthreads();  // red may still be read by a previous call's threads
     red[t] = acc[0];
     __syncthreads();
     for (int h = lanes / 2; h > 0; h >>= 1) {
@@ -142,13 +208,31 @@ __device__ inline void sum_partials(const double *partials, long e0, long e1, do
 
 __global__ __launch_bounds__(STEP_END_THREADS) void k_step_end(Scalars *scalars, const double *partials,
                                                                const int *blk_off,
-                                                               const double *volume, int advance_time)
+                                                               const double *volume, double *chunk,
+                                                               unsigned *tickets, int advance_time)
 {
     __shared__ double red[STEP_END_THREADS];
-    Scalars *sc = &scalars[blockIdx.x];
-    const double sum_volume = volume[blockIdx.x];
-    sum_partials(partials, (long)blk_off[blockIdx.x] * 5, (long)blk_off[blockIdx.x + 1] * 5, red);
+    const int inst = blockIdx.y;
+    const int g0 = blk_off[inst];
+    const int g1 = blk_off[inst + 1];
+    const int nch = std::max(1, (g1 - g0 + STEP_END_CHUNK - 1) / STEP_END_CHUNK);
+    if ((int)blockIdx.x >= nch) return;
+    const int c0 = g0 + (blockIdx.x * STEP_END_CHUNK);
+    const int c1 = std::min(c0 + STEP_END_CHUNK, g1);
+    sum_partials(partials, (long)c0 * 5, (long)c1 * 5, red);
+    if (nch > 1) {
+        // a single chunk's sums equal their sum over that one row: adding the tree's zeros to
+        // sums of squares is exact
+        double *ch = chunk + (chunk_row0(g0, inst, STEP_END_CHUNK) * 5);
+        if (threadIdx.x < 5) {
+            ch[(blockIdx.x * 5) + threadIdx.x] = red[threadIdx.x];
+        }
+        if (!last_workgroup(&tickets[inst], nch)) return;
+        sum_partials(ch, 0, (long)nch * 5, red);
+    }
     if (threadIdx.x == 0) {
+        Scalars *sc = &scalars[inst];
+        const double sum_volume = volume[inst];
         for (int v = 0; v < 5; v++) sc->residual[v] = sqrt(red[v] / sum_volume);
         if (advance_time) {
             sc->time += sc->step;
@@ -156,24 +240,39 @@ __global__ __launch_bounds__(STEP_END_THREADS) void k_step_end(Scalars *scalars,
     }
 }
 
-// out[v] = sqrt(sum V R_v^2 / sum V) from `nb` block partials: k_step_end's sums for one
-// instance, written to a caller's array.  One workgroup.
-__global__ __launch_bounds__(STEP_END_THREADS) void k_residual_out(const double *partials, int nb,
+// out[v] = sqrt(sum V R_v^2 / sum V) from the partials of `ng` groups of 64 rows, in exactly
+// k_step_end's order for one instance, by one workgroup that keeps the chunk sums in shared
+// memory (at most STEP_END_THREADS / 5 chunks) and writes a caller's array.
+constexpr int RESIDUAL_OUT_MAX_GROUPS = STEP_END_CHUNK * (STEP_END_THREADS / 5);
+
+__global__ __launch_bounds__(STEP_END_THREADS) void k_residual_out(const double *partials, int ng,
                                                                    const double *volume, double *out)
 {
     __shared__ double red[STEP_END_THREADS];
-    sum_partials(partials, 0, (long)nb * 5, red);
+    __shared__ double sums[STEP_END_THREADS];
+    const int nch = std::max(1, (ng + STEP_END_CHUNK - 1) / STEP_END_CHUNK);
+    for (int x = 0; x < nch; x++) {
+        const long c0 = (long)x * STEP_END_CHUNK;
+        sum_partials(partials, c0 * 5, std::min<long>(c0 + STEP_END_CHUNK, ng) * 5, red);
+        if (threadIdx.x < 5) {
+            sums[(x * 5) + threadIdx.x] = red[threadIdx.x];
+        }
+    }
+    __syncthreads();
+    if (nch > 1) {
+        sum_partials(sums, 0, (long)nch * 5, red);
+    }
     if (threadIdx.x < 5) {
         out[threadIdx.x] = sqrt(red[threadIdx.x] / volume[0]);
     }
 }
 
-// residual partials of a stored derivative: device rows [5][ni] (perm null), or host rows in
-// the reference layout [rows][5], device row c reading host row perm[c]
+// residual partials of a stored derivative, one per group of 64 rows: device rows [5][ni]
+// (perm null), or host rows in the reference layout [rows][5], device row c reading host row
+// perm[c]
 __global__ void k_residual_partials(const double *__restrict__ R, const int *__restrict__ perm,
                                     const double *__restrict__ vol64, int ni, double *partials)
 {
-    __shared__ double red[BLOCK / 64][5];
     int c = (blockIdx.x * BLOCK) + threadIdx.x;
     double s[5] = {0, 0, 0, 0, 0};
     if (c < ni) {
@@ -186,16 +285,8 @@ __global__ void k_residual_partials(const double *__restrict__ R, const int *__r
     for (int off = 32; off > 0; off >>= 1) {
         for (int v = 0; v < 5; v++) s[v] += __shfl_xor(s[v], off);
     }
-    if ((threadIdx.x & 63) == 0) {
-        for (int v = 0; v < 5; v++) red[threadIdx.x >> 6][v] = s[v];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int v = 0; v < 5; v++) {
-            double b = red[0][v];
-            for (int w = 1; w < BLOCK / 64; w++) b += red[w][v];
-            partials[(blockIdx.x * 5) + v] = b;
-        }
+    if ((threadIdx.x & 63) == 0 && c < ni) {
+        for (int v = 0; v < 5; v++) partials[((long)(c >> 6) * 5) + v] = s[v];
     }
 }
 
@@ -481,7 +572,12 @@ int DevData<S>::build(const Mesh *m, TealGpu *gpu)
     HIPCALL(hipMemset(d_F, 0, sizeof(S) * 10 * (size_t)(nf + 1)));
     HIPCALL(hipMalloc((void **)&d_eps2, sizeof(S) * (size_t)ni));
     HIPCALL(hipMemset(d_eps2, 0, sizeof(S) * (size_t)ni));
-    return build_blocks(inc, ni, flr, area, basis, offl, offr, corr, fcen, crow, ccen);
+    // both block layouts; the residual passes start on the larger one
+    cur = 0;
+    return build_blocks(lay[0], BLOCK, inc, ni, flr, area, basis, offl, offr, corr, fcen, crow,
+                        ccen) ||
+           build_blocks(lay[1], SMALL_BLOCK, inc, ni, flr, area, basis, offl, offr, corr, fcen,
+                        crow, ccen);
 }
 
 // true when the face offsets and corrections equal, bit for bit, their recomputation from
@@ -516,25 +612,29 @@ static bool centres_reproduce(const std::vector<int2> &flr, const std::vector<in
     return true;
 }
 
-// Face lists of the fused stage kernel: block b owns device cells [b*BLOCK, (b+1)*BLOCK); its
-// list holds every face with an incidence in the block, once, in device face order, and
-// slot[k][c] = (list position << 1) | side for incidence k of cell c (-1: none).  The face
-// geometry is copied into list order (faces between blocks twice).
+// Face lists of the fused stage kernel in the layout of `rows` rows per block: block b owns
+// device cells [b*rows, (b+1)*rows); its list holds every face with an incidence in the block,
+// once, in device face order, and slot[k][c] = (list position << 1) | side for incidence k
+// of cell c (-1: none).  The face geometry is copied into list order (faces between blocks
+// twice).
 template <typename S>
-int DevData<S>::build_blocks(const std::vector<int> &inc, int ni, const std::vector<int2> &flr,
-                             const std::vector<double> &area, const std::vector<double> &basis,
-                             const std::vector<double> &offl, const std::vector<double> &offr,
-                             const std::vector<double> &corr, const std::vector<double> &fcen,
-                             const std::vector<int> &crow, const std::vector<double> &ccen)
+int DevData<S>::build_blocks(BlockSet<S> &bl, int rows, const std::vector<int> &inc, int ni,
+                             const std::vector<int2> &flr, const std::vector<double> &area,
+                             const std::vector<double> &basis, const std::vector<double> &offl,
+                             const std::vector<double> &offr, const std::vector<double> &corr,
+                             const std::vector<double> &fcen, const std::vector<int> &crow,
+                             const std::vector<double> &ccen)
 {
     const int nf = (int)area.size();
-    const int nblk = (ni + BLOCK - 1) / BLOCK;
+    const int nblk = (ni + rows - 1) / rows;
     std::vector<int> off(nblk + 1, 0), list, slot((size_t)W * ni, -1);
     std::vector<int> faces;
+    bl.rows = rows;
+    int &max_block_faces = bl.max_block_faces;
     max_block_faces = 0;
     for (int b = 0; b < nblk; b++) {
-        int c0 = b * BLOCK;
-        int c1 = std::min(ni, c0 + BLOCK);
+        int c0 = b * rows;
+        int c1 = std::min(ni, c0 + rows);
         faces.clear();
         for (int c = c0; c < c1; c++) {
             for (int k = 0; k < W; k++) {
@@ -562,11 +662,12 @@ int DevData<S>::build_blocks(const std::vector<int> &inc, int ni, const std::vec
     }
     const int nj = (int)list.size();
     const size_t cap = (size_t)std::max(nj, 1);
-    num_block_faces = nj;
+    bl.num_block_faces = nj;
     // FP64: the centre form (48 fewer bytes per face) where it reproduces the stored geometry.
     // FP32 keeps the offsets rounded from FP64 (a difference of FP32 centres loses digits) but
     // derives the correction from them and the tangents from the normal (40 fewer bytes).
-    centre_form = sizeof(S) == 8 && centres_reproduce(flr, crow, fcen, ccen, offl, offr, corr);
+    bl.centre_form = sizeof(S) == 8 && centres_reproduce(flr, crow, fcen, ccen, offl, offr, corr);
+    const bool centre_form = bl.centre_form;
     const bool fp64 = sizeof(S) == 8;
     const int brows = fp64 ? 9 : 3;
     std::vector<int2> lflr(cap);
@@ -600,7 +701,7 @@ int DevData<S>::build_blocks(const std::vector<int> &inc, int ni, const std::vec
             for (int k = 0; k < 4; k++) lcorr[((size_t)k * nj) + j] = corr[((size_t)k * nf) + f];
         }
     }
-    if (upload(&d_boff, off) || upload(&d_bslot, slot) || upload(&d_bflr, lflr)) {
+    if (upload(&bl.d_boff, off) || upload(&bl.d_bslot, slot) || upload(&bl.d_bflr, lflr)) {
         return -1;
     }
     if (!fp64) {  // (area, nx), (ny, nz), (olx, oly), (olz, orx), (ory, orz) per face
@@ -621,7 +722,7 @@ int DevData<S>::build_blocks(const std::vector<int> &inc, int ni, const std::vec
                 pairs[((((size_t)q * cap) + j) * 2) + 1] = ten[(2 * q) + 1];
             }
         }
-        return upload_conv(&d_bpairs, pairs);
+        return upload_conv(&bl.d_bpairs, pairs);
     }
     if (centre_form) {  // (area, b0), (b1, b2), ..., (b7, b8), (fcx, fcy), (fcz, 0) per face
         std::vector<double> pairs(14 * cap, 0.0);
@@ -634,14 +735,15 @@ int DevData<S>::build_blocks(const std::vector<int> &inc, int ni, const std::vec
                 pairs[((((size_t)q * cap) + j) * 2) + 1] = rec[(2 * q) + 1];
             }
         }
-        if (upload(&d_bcrow, lcrow) || upload_conv(&d_bpairs, pairs) || upload_conv(&d_ccen, ccen)) {
+        if (upload(&bl.d_bcrow, lcrow) || upload_conv(&bl.d_bpairs, pairs) ||
+            (!d_ccen && upload_conv(&d_ccen, ccen))) {
             return -1;
         }
         return 0;
     }
-    if (upload_conv(&d_barea, larea) || upload_conv(&d_bbasis, lbasis) ||
-        upload_conv(&d_boffl, loffl) || upload_conv(&d_boffr, loffr) ||
-        upload_conv(&d_bcorr, lcorr)) {
+    if (upload_conv(&bl.d_barea, larea) || upload_conv(&bl.d_bbasis, lbasis) ||
+        upload_conv(&bl.d_boffl, loffl) || upload_conv(&bl.d_boffr, loffr) ||
+        upload_conv(&bl.d_bcorr, lcorr)) {
         return -1;
     }
     return 0;
@@ -650,14 +752,16 @@ int DevData<S>::build_blocks(const std::vector<int> &inc, int ni, const std::vec
 template <typename S>
 BlockTopo DevData<S>::block_topo() const
 {
-    return BlockTopo{d_boff, d_bslot, d_bflr, num_block_faces, max_block_faces};
+    const BlockSet<S> &b = lay[cur];
+    return BlockTopo{b.d_boff, b.d_bslot, b.d_bflr, b.num_block_faces, b.max_block_faces};
 }
 
 template <typename S>
 Geom<S> DevData<S>::block_geom() const
 {
-    return Geom<S>{d_vol,    d_proj,   d_eps2,   d_w,     d_barea,  d_bbasis,
-                   d_boffl,  d_boffr,  d_bcorr,  d_bfcen, d_ccen,   d_bcrow, d_bpairs};
+    const BlockSet<S> &b = lay[cur];
+    return Geom<S>{d_vol,     d_proj,     d_eps2,    d_w,     b.d_barea,  b.d_bbasis, b.d_boffl,
+                   b.d_boffr, b.d_bcorr,  b.d_bfcen, d_ccen,  b.d_bcrow,  b.d_bpairs};
 }
 
 template <typename S>
@@ -665,12 +769,21 @@ void DevData<S>::release()
 {
     void *ptrs[] = {d_inc, d_nbr, d_flr, d_w, d_vol, d_proj, d_area, d_basis, d_offl,
                     d_offr, d_corr, d_vol64, d_bleft, d_bright, d_bface, d_bkind, d_bctx,
-                    d_P, d_P2, d_GP, d_F, d_eps2, d_boff, d_bslot, d_bflr, d_barea, d_bbasis,
-                    d_boffl, d_boffr, d_bcorr, d_bfcen, d_bcrow, d_ccen, d_bpairs};
+                    d_P, d_P2, d_GP, d_F, d_eps2, d_ccen};
     for (void *p : ptrs) {
         if (p) {
             (void)hipFree(p);
         }
+    }
+    for (BlockSet<S> &b : lay) {
+        void *bp[] = {b.d_boff,  b.d_bslot, b.d_bflr,  b.d_barea, b.d_bbasis, b.d_boffl,
+                      b.d_boffr, b.d_bcorr, b.d_bfcen, b.d_bcrow, b.d_bpairs};
+        for (void *p : bp) {
+            if (p) {
+                (void)hipFree(p);
+            }
+        }
+        b = BlockSet<S>();
     }
 }
 
@@ -718,13 +831,18 @@ static TealGpu *create_handle(const Mesh *mesh, int precision, int ninst, const 
         g->d32.fused = g->d32.block_lds() > 0 && g->d32.block_lds() <= (size_t)g->lds_max;
     }
     g->nblocks = grid_of(g->ni);
-    std::vector<int> blk_inst(g->nblocks, 0);
+    // partial sums per group of 64 rows; a batch pads every instance to whole blocks, so an
+    // instance's groups are its blocks' groups
+    constexpr int GPB = BLOCK / 64;  // groups per block
+    g->ngroups = (g->ni + 63) / 64;
+    std::vector<int> grp_inst(std::max(g->ngroups, 1), 0), inst_grp_off(ninst + 1, 0);
     for (int i = 0; i < ninst; i++) {
-        for (int b = g->inst_blk_off[i]; b < g->inst_blk_off[i + 1]; b++) blk_inst[b] = i;
+        inst_grp_off[i + 1] = std::min(g->ngroups, g->inst_blk_off[i + 1] * GPB);
+        for (int q = inst_grp_off[i]; q < inst_grp_off[i + 1]; q++) grp_inst[q] = i;
     }
-    std::vector<int> all_blk = {0, g->nblocks};
-    if (rc == 0 && (upload(&g->d_cell_perm, g->cell_perm) || upload(&g->d_blk_inst, blk_inst) ||
-                    upload(&g->d_inst_blk_off, g->inst_blk_off) || upload(&g->d_all_blk, all_blk) ||
+    std::vector<int> all_grp = {0, g->ngroups};
+    if (rc == 0 && (upload(&g->d_cell_perm, g->cell_perm) || upload(&g->d_grp_inst, grp_inst) ||
+                    upload(&g->d_inst_grp_off, inst_grp_off) || upload(&g->d_all_grp, all_grp) ||
                     upload(&g->d_inst_volume, g->inst_volume))) {
         rc = -1;
     }
@@ -736,14 +854,23 @@ static TealGpu *create_handle(const Mesh *mesh, int precision, int ninst, const 
         teal_gpu_set_error("teal_gpu_create: hipEventCreate failed");
         rc = -1;
     }
+    // chunk results of the per-step reductions (see chunk_row0), and their tickets
+    g->max_inst_groups = 0;
+    for (int i = 0; i < ninst; i++) {
+        g->max_inst_groups = std::max(g->max_inst_groups, inst_grp_off[i + 1] - inst_grp_off[i]);
+    }
+    const size_t chunk_rows = (size_t)g->ngroups / STEP_END_CHUNK + ninst + 1;
     // host-layout staging: up to 15 values per mesh row, or 5 per device row
     size_t stage = std::max((size_t)15 * mesh->cells.num, (size_t)5 * g->nc);
     if (rc == 0 &&
         (hipMalloc((void **)&g->d_stage, sizeof(double) * stage) != hipSuccess ||
          hipMalloc((void **)&g->d_U0, sizeof(double) * 5 * (size_t)g->ni) != hipSuccess ||
          hipMalloc((void **)&g->d_R, sizeof(double) * 5 * (size_t)g->ni) != hipSuccess ||
-         hipMalloc((void **)&g->d_partials, sizeof(double) * 5 * (size_t)g->nblocks) != hipSuccess ||
-         hipMalloc((void **)&g->d_dtpart, sizeof(double) * (size_t)g->nblocks) != hipSuccess ||
+         hipMalloc((void **)&g->d_partials, sizeof(double) * 5 * (size_t)std::max(g->ngroups, 1)) !=
+             hipSuccess ||
+         hipMalloc((void **)&g->d_dtpart, sizeof(double) * (size_t)std::max(g->ngroups, 1)) != hipSuccess ||
+         hipMalloc((void **)&g->d_chunk, sizeof(double) * 5 * chunk_rows) != hipSuccess ||
+         hipMalloc((void **)&g->d_tickets, sizeof(unsigned) * (size_t)ninst) != hipSuccess ||
          hipMalloc((void **)&g->d_scalars, sizeof(Scalars) * (size_t)ninst) != hipSuccess ||
          hipHostMalloc((void **)&g->h_scalars, sizeof(Scalars) * (size_t)ninst, hipHostMallocDefault) !=
              hipSuccess)) {
@@ -753,6 +880,9 @@ static TealGpu *create_handle(const Mesh *mesh, int precision, int ninst, const 
     if (rc == 0) {
         hipLaunchKernelGGL(k_init_scalars, dim3(ninst), dim3(1), 0, g->stream, g->d_scalars,
                            (const double *)nullptr, 0.0);
+        if (hipMemsetAsync(g->d_tickets, 0, sizeof(unsigned) * (size_t)ninst, g->stream) != hipSuccess) {
+            rc = -1;
+        }
         if (hipStreamSynchronize(g->stream) != hipSuccess) {
             teal_gpu_set_error("teal_gpu_create: initialization kernel failed");
             rc = -1;
@@ -852,8 +982,9 @@ extern "C" void teal_gpu_destroy(TealGpu *g)
     }
     g->d64.release();
     g->d32.release();
-    void *ptrs[] = {g->d_stage, g->d_U0, g->d_R, g->d_partials, g->d_dtpart, g->d_scalars,
-                    g->d_cell_perm, g->d_blk_inst, g->d_inst_blk_off, g->d_all_blk,
+    void *ptrs[] = {g->d_stage,     g->d_U0,        g->d_R,          g->d_partials,
+                    g->d_dtpart,    g->d_chunk,     g->d_tickets,    g->d_scalars,
+                    g->d_cell_perm, g->d_grp_inst,  g->d_inst_grp_off, g->d_all_grp,
                     g->d_inst_volume};
     for (void *p : ptrs) {
         if (p) {
@@ -1018,7 +1149,8 @@ int teal_residual_pass(TealGpu *g, DevData<S> &d, int mode, double alpha, int la
     if (g->cfg.space_order == 2) {
         bool timed = g->timing && g->ev_grad.begin(g->stream);
         // inviscid passes keep only the limited gradient (15 values, in rows of 16)
-        Kern<S>::grad_limit(g->stream, g->cfg.limiter, g->cfg.viscous ? 0 : 2, g->topo(), d.geom(),
+        Kern<S>::grad_limit(g->stream, g->cfg.limiter, g->cfg.viscous ? 0 : 2, g->grad_block,
+                            g->topo(), d.geom(),
                             d.d_P, d.d_GP);
         if (timed) {
             g->ev_grad.end(g->stream);
@@ -1033,7 +1165,7 @@ int teal_residual_pass(TealGpu *g, DevData<S> &d, int mode, double alpha, int la
     a.U0_next = g->d_U0;
     a.alpha = alpha;
     a.scalars = g->d_scalars;
-    a.blk_inst = (g->ninst > 1) ? g->d_blk_inst : nullptr;
+    a.grp_inst = (g->ninst > 1) ? g->d_grp_inst : nullptr;
     a.partials = g->d_partials;
     a.dtpart = g->d_dtpart;
     a.keep_u = (g->precision == TEAL_MIXED);
@@ -1041,7 +1173,8 @@ int teal_residual_pass(TealGpu *g, DevData<S> &d, int mode, double alpha, int la
     if (d.fused) {  // one kernel: face fluxes in LDS, accumulation, stage update into d_P2
         bool timed = g->timing && g->ev_flux.begin(g->stream);
         int rc = Kern<S>::flux_block(g->stream, g->cfg.flux, g->cfg.viscous, g->cfg.space_order, kmode,
-                                     d.block_lds(g->cfg.viscous, conv), g->topo(), d.block_topo(),
+                                     d.block_rows(), d.block_lds(g->cfg.viscous, conv), g->topo(),
+                                     d.block_topo(),
                                      d.block_geom(),
                                      g->phys(),
                                      d.d_P, d.d_GP, d.d_P2, a);
@@ -1229,7 +1362,7 @@ static int gradient_impl(TealGpu *g, DevData<S> &d, double *primitive, double *g
         HIPCALL(hipMemcpyAsync(g->d_stage, gradient, bytes, hipMemcpyHostToDevice, g->stream));
         Kern<S>::to_device(g->stream, g->d_stage, d.d_GP, g->ni, 15, 15, 20, g->d_cell_perm, 0);
     }
-    Kern<S>::grad_limit(g->stream, lim, mode, g->topo(), d.geom(), d.d_P, d.d_GP);
+    Kern<S>::grad_limit(g->stream, lim, mode, g->grad_block, g->topo(), d.geom(), d.d_P, d.d_GP);
     HIPCALL(hipGetLastError());
     // rows of 20: gradient (15) then psi (5); the limited gradient is G * psi (limiter.c:61-63)
     std::vector<double> GP((size_t)20 * g->mesh_ni);
@@ -1274,8 +1407,9 @@ int teal_timestep_pass(TealGpu *g, DevData<S> &d, int write_u0, double *out)
     HIPCALL(hipGetLastError());
     if (out) {
         // one minimum over all rows (a batch handle answers for the union of its meshes)
-        hipLaunchKernelGGL(k_dt_min, dim3(1), dim3(DT_THREADS), 0, g->stream, g->d_scalars,
-                           (const double *)g->d_dtpart, (const int *)g->d_all_blk, 0, 0.0, 0.0);
+        hipLaunchKernelGGL(k_dt_min, dim3(chunks_of(g->ngroups, DT_CHUNK), 1), dim3(DT_THREADS), 0,
+                           g->stream, g->d_scalars, (const double *)g->d_dtpart,
+                           (const int *)g->d_all_grp, g->d_chunk, g->d_tickets, 0, 0.0, 0.0);
         HIPCALL(hipMemcpyAsync(g->h_scalars, g->d_scalars, sizeof(Scalars), hipMemcpyDeviceToHost,
                                g->stream));
         HIPCALL(hipStreamSynchronize(g->stream));
@@ -1329,9 +1463,10 @@ int teal_stream_wait(TealGpu *g)
 // per-instance residual norms of the partials in d_partials: residual[5 * i + v]
 static int read_residuals(TealGpu *g, double *residual)
 {
-    hipLaunchKernelGGL(k_step_end, dim3(g->ninst), dim3(STEP_END_THREADS), 0, g->stream,
-                       g->d_scalars, (const double *)g->d_partials, (const int *)g->d_inst_blk_off,
-                       (const double *)g->d_inst_volume, 0);
+    hipLaunchKernelGGL(k_step_end, dim3(chunks_of(g->max_inst_groups, STEP_END_CHUNK), g->ninst),
+                       dim3(STEP_END_THREADS), 0, g->stream, g->d_scalars,
+                       (const double *)g->d_partials, (const int *)g->d_inst_grp_off,
+                       (const double *)g->d_inst_volume, g->d_chunk, g->d_tickets, 0);
     HIPCALL(hipGetLastError());
     HIPCALL(hipMemcpyAsync(g->h_scalars, g->d_scalars, sizeof(Scalars) * (size_t)g->ninst,
                            hipMemcpyDeviceToHost, g->stream));
@@ -1366,14 +1501,15 @@ extern "C" int teal_gpu_residual(TealGpu *g, const double *derivative, double *r
 }
 
 // ------------------------------------------------------------------ residual, no allocation
-// teal_gpu_residual_noalloc keeps its block partial sums in this fixed array rather than in
-// memory it allocates.  In host code the name `scratch` is not a device address: the address
+// teal_gpu_residual_noalloc keeps its partial sums (one per 64 rows) in this fixed array rather
+// than in memory it allocates.  In host code the name `scratch` is not a device address: the address
 // of the current device's copy comes from hipGetSymbolAddress, once per device, and only
 // that address is passed to kernels.  Every call on a device shares its copy; each call
 // overwrites the partials it reads (nothing has to start at zero), and calls that overlap on
 // one device race on it.
-constexpr int NOALLOC_MAX_BLOCKS = 65536;  // inner cells <= 65536 * BLOCK (16,777,216)
-static __device__ __align__(256) double scratch[(size_t)NOALLOC_MAX_BLOCKS * 5];  // 2.5 MiB
+constexpr int NOALLOC_MAX_GROUPS = 262144;  // inner cells <= 262144 * 64 (16,777,216)
+static __device__ __align__(256) double scratch[(size_t)NOALLOC_MAX_GROUPS * 5];  // 10 MiB
+static_assert(NOALLOC_MAX_GROUPS <= RESIDUAL_OUT_MAX_GROUPS, "k_residual_out keeps its chunk sums in LDS");
 constexpr int NOALLOC_MAX_DEVICES = 64;
 static double *scratch_dev[NOALLOC_MAX_DEVICES];
 
@@ -1407,9 +1543,9 @@ extern "C" int teal_gpu_residual_noalloc(TealGpu *g, const double *derivative, d
         teal_gpu_set_error("residual_noalloc: null device array");
         return -1;
     }
-    if (g->nblocks > NOALLOC_MAX_BLOCKS) {
+    if (g->ngroups > NOALLOC_MAX_GROUPS) {
         teal_gpu_set_error("residual_noalloc: %d inner cells exceed the limit of %ld", g->ni,
-                           (long)NOALLOC_MAX_BLOCKS * BLOCK);
+                           (long)NOALLOC_MAX_GROUPS * 64);
         return -1;
     }
     double *part = nullptr;
@@ -1419,7 +1555,7 @@ extern "C" int teal_gpu_residual_noalloc(TealGpu *g, const double *derivative, d
     hipLaunchKernelGGL(k_residual_partials, dim3(g->nblocks), dim3(BLOCK), 0, s, derivative,
                        (const int *)g->d_cell_perm, vol64, g->ni, part);
     hipLaunchKernelGGL(k_residual_out, dim3(1), dim3(STEP_END_THREADS), 0, s, (const double *)part,
-                       g->nblocks, (const double *)g->d_inst_volume, residual);
+                       g->ngroups, (const double *)g->d_inst_volume, residual);
     HIPCALL(hipGetLastError());
     return 0;
 }
@@ -1496,15 +1632,17 @@ template <typename S>
 static int explicit_stages(TealGpu *g, DevData<S> &d, double max_step, double courant,
                            const double *alpha, int num_stages)
 {
-    hipLaunchKernelGGL(k_dt_min, dim3(g->ninst), dim3(DT_THREADS), 0, g->stream, g->d_scalars,
-                       (const double *)g->d_dtpart, (const int *)g->d_inst_blk_off, 1, courant,
+    hipLaunchKernelGGL(k_dt_min, dim3(chunks_of(g->max_inst_groups, DT_CHUNK), g->ninst),
+                       dim3(DT_THREADS), 0, g->stream, g->d_scalars, (const double *)g->d_dtpart,
+                       (const int *)g->d_inst_grp_off, g->d_chunk, g->d_tickets, 1, courant,
                        max_step);
     for (int s = 0; s < num_stages; s++) {
         if (teal_residual_pass(g, d, 1, alpha[s + 1], s == num_stages - 1)) return -1;
     }
-    hipLaunchKernelGGL(k_step_end, dim3(g->ninst), dim3(STEP_END_THREADS), 0, g->stream,
-                       g->d_scalars, (const double *)g->d_partials, (const int *)g->d_inst_blk_off,
-                       (const double *)g->d_inst_volume, 1);
+    hipLaunchKernelGGL(k_step_end, dim3(chunks_of(g->max_inst_groups, STEP_END_CHUNK), g->ninst),
+                       dim3(STEP_END_THREADS), 0, g->stream, g->d_scalars,
+                       (const double *)g->d_partials, (const int *)g->d_inst_grp_off,
+                       (const double *)g->d_inst_volume, g->d_chunk, g->d_tickets, 1);
     HIPCALL(hipGetLastError());
     return 0;
 }
@@ -1883,50 +2021,83 @@ template <typename S>
 static int autotune(TealGpu *g, DevData<S> &d, int steps, double courant, const double *alpha,
                     int num_stages, double *ms)
 {
-    const bool can_fuse = d.block_lds() > 0 && d.block_lds() <= (size_t)g->lds_max;
+    auto fits = [&](int layout) {
+        const int keep = d.cur;
+        d.cur = layout;
+        const size_t lds = d.block_lds();
+        d.cur = keep;
+        return d.lay[layout].rows > 0 && lds > 0 && lds <= (size_t)g->lds_max;
+    };
     Snapshot<S> snap;
     if (snap.save(g, d)) return -1;
-    const int graphs0 = g->graphs;
+    const int graphs0 = g->graphs, cur0 = d.cur, gb0 = g->grad_block;
     hipEvent_t t0, t1;
     HIPCALL(hipEventCreate(&t0));
     HIPCALL(hipEventCreate(&t1));
-    double best = INFINITY;
-    int best_shape = -1;
-    int rc = 0;
-    for (int shape = 0; shape < 4 && rc == 0; shape++) {
-        const bool fused = shape < 2;
-        ms[shape] = NAN;
-        if (fused && !can_fuse) continue;
-        d.fused = fused;
-        g->graphs = shape & 1;
+    // ms per step of the current configuration, from the saved state
+    auto time_steps = [&](double *out) -> int {
         drop_graph(g);
-        rc = snap.restore(g, d) || run_steps(g, d, 1, courant, alpha, num_stages) ||
-             (g->graphs && step_graph(g, d, courant, alpha, num_stages));
-        if (rc) break;
+        if (snap.restore(g, d) || run_steps(g, d, 1, courant, alpha, num_stages) ||
+            (g->graphs && step_graph(g, d, courant, alpha, num_stages))) {
+            return -1;
+        }
         (void)hipEventRecord(t0, g->stream);
-        rc = run_steps(g, d, steps, courant, alpha, num_stages);
+        int rc = run_steps(g, d, steps, courant, alpha, num_stages);
         (void)hipEventRecord(t1, g->stream);
         float t = 0;
         if (rc || hipEventSynchronize(t1) != hipSuccess || hipEventElapsedTime(&t, t0, t1) != hipSuccess) {
-            rc = -1;
-            break;
+            return -1;
         }
-        ms[shape] = t / steps;
-        if (ms[shape] < best) {
+        *out = t / steps;
+        return 0;
+    };
+    double best = INFINITY;
+    int best_shape = -1;
+    int rc = 0;
+    // stage shapes: fused with BLOCK-row blocks, fused with SMALL_BLOCK-row blocks, split;
+    // each with eager launches, then with graph replay
+    for (int shape = 0; shape < TEAL_AUTOTUNE_STAGE_SHAPES && rc == 0; shape++) {
+        const bool fused = shape < 4;
+        const int layout = (shape < 2) ? 0 : 1;
+        ms[shape] = NAN;
+        if (fused && !fits(layout)) continue;
+        d.fused = fused;
+        d.cur = fused ? layout : cur0;
+        g->graphs = shape & 1;
+        rc = time_steps(&ms[shape]);
+        if (rc == 0 && ms[shape] < best) {
             best = ms[shape];
             best_shape = shape;
         }
     }
+    if (rc == 0 && best_shape >= 0) {
+        d.fused = best_shape < 4;
+        d.cur = (best_shape < 2) ? 0 : (best_shape < 4) ? 1 : cur0;
+        g->graphs = best_shape & 1;
+        // gradient kernel workgroup sizes on the chosen stage shape
+        static const int sizes[TEAL_AUTOTUNE_SHAPES - TEAL_AUTOTUNE_STAGE_SHAPES] = {BLOCK, 128, 64};
+        double best_gb = INFINITY;
+        int pick = gb0;
+        for (int k = 0; k < TEAL_AUTOTUNE_SHAPES - TEAL_AUTOTUNE_STAGE_SHAPES && rc == 0; k++) {
+            double *out = &ms[TEAL_AUTOTUNE_STAGE_SHAPES + k];
+            *out = NAN;
+            g->grad_block = sizes[k];
+            rc = time_steps(out);
+            if (rc == 0 && *out < best_gb) {
+                best_gb = *out;
+                pick = sizes[k];
+            }
+        }
+        g->grad_block = (rc == 0) ? pick : gb0;
+    }
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     drop_graph(g);
-    if (rc == 0 && best_shape >= 0) {
-        d.fused = best_shape < 2;
-        g->graphs = best_shape & 1;
-    }
-    else {
-        d.fused = can_fuse;
+    if (rc != 0 || best_shape < 0) {
+        d.cur = cur0;
+        d.fused = fits(cur0);
         g->graphs = graphs0;
+        g->grad_block = gb0;
     }
     if (snap.restore(g, d)) return -1;
     return rc;
@@ -1942,10 +2113,48 @@ extern "C" int teal_gpu_autotune(TealGpu *g, int steps, double courant, int time
         teal_gpu_set_error("autotune: at least 2 timed steps (%d)", steps);
         return -1;
     }
-    double tmp[4];
+    double tmp[TEAL_AUTOTUNE_SHAPES];
     double *out = ms ? ms : tmp;
     return (g->precision == TEAL_FP64) ? autotune(g, g->d64, steps, courant, alpha, num_stages, out)
                                        : autotune(g, g->d32, steps, courant, alpha, num_stages, out);
+}
+
+// one of the autotuner's stage shapes (0 .. TEAL_AUTOTUNE_STAGE_SHAPES - 1) and a gradient
+// workgroup size, set directly
+template <typename S>
+static int set_shape(TealGpu *g, DevData<S> &d, int shape, int grad_block)
+{
+    const bool fused = shape < 4;
+    const int layout = (shape < 2) ? 0 : 1;
+    if (fused) {
+        const int keep = d.cur;
+        d.cur = layout;
+        const size_t lds = d.block_lds();
+        if (d.lay[layout].rows == 0 || lds == 0 || lds > (size_t)g->lds_max) {
+            d.cur = keep;
+            teal_gpu_set_error("set_shape: the %d-row block layout does not fit (%zu bytes of LDS)",
+                               d.lay[layout].rows, lds);
+            return -1;
+        }
+    }
+    d.fused = fused;
+    g->graphs = shape & 1;
+    g->grad_block = grad_block;
+    drop_graph(g);
+    return 0;
+}
+
+extern "C" int teal_gpu_set_shape(TealGpu *g, int shape, int grad_block)
+{
+    if (check_ready(g)) return -1;
+    if (shape < 0 || shape >= TEAL_AUTOTUNE_STAGE_SHAPES ||
+        (grad_block != 64 && grad_block != 128 && grad_block != BLOCK)) {
+        teal_gpu_set_error("set_shape: invalid shape (%d) or gradient workgroup (%d)", shape,
+                           grad_block);
+        return -1;
+    }
+    return (g->precision == TEAL_FP64) ? set_shape(g, g->d64, shape, grad_block)
+                                       : set_shape(g, g->d32, shape, grad_block);
 }
 
 // ------------------------------------------------------------------ timing
@@ -2034,9 +2243,14 @@ extern "C" int teal_gpu_info(TealGpu *g, long *counts)
     counts[4] = g->nfb;
     counts[5] = W;
     const bool fused = (g->precision == TEAL_FP64) ? g->d64.fused : g->d32.fused;
-    const bool centre = (g->precision == TEAL_FP64) ? g->d64.centre_form : g->d32.centre_form;
+    const bool centre = (g->precision == TEAL_FP64) ? g->d64.lay[g->d64.cur].centre_form
+                                                    : g->d32.lay[g->d32.cur].centre_form;
+    const int rows = (g->precision == TEAL_FP64) ? g->d64.block_rows() : g->d32.block_rows();
     counts[6] = fused;
     counts[7] = fused && centre;
+    counts[8] = fused ? rows : 0;
+    counts[9] = g->grad_block;
+    counts[10] = teal_krylov_column_ept(g);
     return 0;
 }
 

@@ -27,14 +27,16 @@
 // Kernels per residual evaluation:
 //   k_boundary   ghost states of boundary faces (src/equations/boundary.c:5-43)
 //   k_grad_limit LSQ gradient (gradient.c:7-58) fused with the limiter (limiter.c:23-123);
-//                one thread per inner cell
-//   k_flux_block one workgroup per block of 256 device cells: the viscous and convective flux
+//                one thread per inner cell, workgroups of 64, 128 or 256 (autotuned)
+//   k_flux_block one workgroup per block of 256 or 128 device cells (autotuned block layout):
+//                the viscous and convective flux
 //                of every face touching the block (derivative.c:91-227) into LDS, then per
 //                cell the viscous contributions in face order, the convective ones (FP32:
 //                the merged records, summed in FP64), R = -R/V (derivative.c:229-267) and, in
 //                RK mode, the stage update + cons2prim (src/simulation/advance.c:101-109);
-//                the last stage also produces the next step's U0, the dt minimum and
-//                residual partial sums
+//                the last stage also produces the next step's U0, the dt minima and the
+//                residual partial sums, one per 64 device rows (so the sums do not depend on
+//                the block layout)
 //   k_face + k_accum  the same work as two kernels through a face flux array in HBM, used
 //                when a block's face list does not fit in LDS
 #pragma once
@@ -51,15 +53,10 @@ constexpr int W = 6;        // MAX_CELL_FACES (src/mesh.h:8-12)
 #define TEAL_BLOCK 256
 #endif
 constexpr int BLOCK = TEAL_BLOCK;
-// threads of the fused stage kernel per block of BLOCK cells (>= BLOCK): the face phase
-// spreads a block's faces over them, the cell phase uses the first BLOCK.  Measured on TGV
-// 128^3 with 256-cell blocks: 256 threads beat 512 (0.62 vs 0.75 ms FP64), more resident
-// waves only enlarge the gathered working set.
-#ifndef TEAL_FLUX_THREADS
-#define TEAL_FLUX_THREADS TEAL_BLOCK
-#endif
-constexpr int FLUX_THREADS = TEAL_FLUX_THREADS;
-static_assert(FLUX_THREADS >= BLOCK && FLUX_THREADS % 64 == 0, "stage kernel threads");
+// Device rows are padded (batches) and numbered in blocks of BLOCK; the fused stage kernel runs
+// one workgroup per block of BS = BLOCK or BLOCK / 2 rows (two block layouts, one thread per
+// row: the face phase spreads the block's faces over the same threads).
+constexpr int SMALL_BLOCK = BLOCK / 2;
 constexpr int NUM_XCD = 8;
 
 struct Topo {
@@ -246,11 +243,11 @@ __global__ __launch_bounds__(BLOCK) void k_boundary(int num, int nf, const int *
 // gradient alone): compute and limit, GP rows [ni][16] = gradient * psi (limiter.c:61-63)
 // and one unused value, so rows stay 16-byte aligned.
 // WK: incidence rows visited (W, or 4 on meshes of tetrahedra, whose rows 4 and 5 are empty).
-template <typename S, int LIM, int MODE, int WK = W>
-__global__ __launch_bounds__(BLOCK) void k_grad_limit(Topo t, Geom<S> g, const S *__restrict__ P,
-                                                      S *__restrict__ GP)
+template <typename S, int LIM, int MODE, int WK = W, int GB = BLOCK>
+__global__ __launch_bounds__(GB) void k_grad_limit(Topo t, Geom<S> g, const S *__restrict__ P,
+                                                   S *__restrict__ GP)
 {
-    int c = (xcd_block() * BLOCK) + threadIdx.x;
+    int c = (xcd_block() * GB) + threadIdx.x;
     if (c >= t.ni) {
         return;
     }
@@ -565,9 +562,9 @@ struct StageArgs {
     double *U0_next;      // MODE 2: next step's U0 (may alias U0)
     double alpha;         // stage coefficient alpha[i+1]
     Scalars *scalars;     // step, dt reduction: one record per instance
-    const int *blk_inst;  // instance of each block of rows (null: one instance)
-    double *partials;     // MODE 2: [num_blocks][5] residual partial sums
-    double *dtpart;       // MODE 2: [num_blocks] minima of the local time step
+    const int *grp_inst;  // instance of each group of 64 rows (null: one instance)
+    double *partials;     // MODE 2: [num_groups][5] residual partial sums, one per 64 rows
+    double *dtpart;       // MODE 2: [num_groups] minima of the local time step
     int keep_u;           // MODE 2: next U0 = U (mixed) instead of prim2cons(P_new) (FP64)
     // bulk-velocity forcing source, or null: {entity average of the primitive state (5),
     // target velocity U, relaxation time tau}; the cell's source is s1 = rho (U - <u>) / tau
@@ -578,13 +575,13 @@ struct StageArgs {
 
 // Stage epilogue of inner cell c (active) given its accumulated face sums: finalize
 // R = -R/V (derivative.c:253-258); MODE 0 writes R; MODE 1 writes P_new = cons2prim(U0 +
-// (R alpha) step) (advance.c:103-108); MODE 2 also produces the next step's U0, the dt minimum
-// and the residual partial sums of this workgroup (block-wide: all threads must call it).
+// (R alpha) step) (advance.c:103-108); MODE 2 also produces the next step's U0, and the dt
+// minimum and residual partial sums of the 64 rows of this wavefront (c >> 6: every lane of
+// the wave must call it, with consecutive c).
 // Pin is the state the residual is taken of (the source term's context); it may equal Pout.
-template <typename S, int MODE, int NT = BLOCK>
+template <typename S, int MODE>
 __device__ inline void stage_epilogue(const Topo &t, const Geom<S> &g, const Phys &ph, const Acc<S> &acc,
-                                      int c, bool active, int blk, const S *Pin, S *Pout,
-                                      const StageArgs &a)
+                                      int c, bool active, const S *Pin, S *Pout, const StageArgs &a)
 {
     const int ni = t.ni;
     double r[5] = {0, 0, 0, 0, 0};
@@ -610,7 +607,7 @@ __device__ inline void stage_epilogue(const Topo &t, const Geom<S> &g, const Phy
             for (int v = 0; v < 5; v++) a.R[((long)v * ni) + c] = r[v];
         }
         else {
-            double step = a.scalars[a.blk_inst ? a.blk_inst[blk] : 0].step;
+            double step = a.scalars[a.grp_inst ? a.grp_inst[c >> 6] : 0].step;
             for (int v = 0; v < 5; v++) unew[v] = ld_stream(&a.U0[((long)v * ni) + c]) + ((r[v] * a.alpha) * step);
             if constexpr (sizeof(S) == 8) {
                 cons2prim<double>(pnew, unew, ph.gamma);
@@ -622,7 +619,6 @@ __device__ inline void stage_epilogue(const Topo &t, const Geom<S> &g, const Phy
         }
     }
     if (MODE == 2) {
-        __shared__ double red[NT / 64][6];
         double dt = __longlong_as_double(0x7ff0000000000000LL);  // +inf
         double sums[5] = {0, 0, 0, 0, 0};
         if (active) {
@@ -645,28 +641,15 @@ __device__ inline void stage_epilogue(const Topo &t, const Geom<S> &g, const Phy
                 sums[v] = vol * (r[v] * r[v]);
             }
         }
-        // deterministic block reduction: wavefront butterfly, then LDS
+        // deterministic wavefront butterfly; lane 0 (row c = the group's first) writes
         for (int off = 32; off > 0; off >>= 1) {
             dt = fmin(dt, __shfl_xor(dt, off));
             for (int v = 0; v < 5; v++) sums[v] += __shfl_xor(sums[v], off);
         }
-        int lane = threadIdx.x & 63;
-        int wave = threadIdx.x >> 6;
-        if (lane == 0) {
-            red[wave][0] = dt;
-            for (int v = 0; v < 5; v++) red[wave][v + 1] = sums[v];
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double bdt = red[0][0];
-            double bs[5];
-            for (int v = 0; v < 5; v++) bs[v] = red[0][v + 1];
-            for (int w = 1; w < NT / 64; w++) {
-                bdt = fmin(bdt, red[w][0]);
-                for (int v = 0; v < 5; v++) bs[v] += red[w][v + 1];
-            }
-            a.dtpart[blk] = bdt;
-            for (int v = 0; v < 5; v++) a.partials[(blk * 5) + v] = bs[v];
+        if ((threadIdx.x & 63) == 0 && c < ni) {
+            const long grp = c >> 6;
+            a.dtpart[grp] = dt;
+            for (int v = 0; v < 5; v++) a.partials[(grp * 5) + v] = sums[v];
         }
     }
 }
@@ -740,11 +723,11 @@ __global__ __launch_bounds__(BLOCK) void k_accum(Topo t, Geom<S> g, Phys ph, con
             for (int v = 0; v < 5; v++) x[v] = rec[v];
         });
     }
-    stage_epilogue<S, MODE>(t, g, ph, acc, c, active, blk, Pout, Pout, a);
+    stage_epilogue<S, MODE>(t, g, ph, acc, c, active, Pout, Pout, a);
 }
 
 // Per-block face lists of the fused kernel: the workgroup of device cells
-// [b*BLOCK, (b+1)*BLOCK) computes the fluxes of every face touching them into LDS, then each
+// [b*BS, (b+1)*BS) computes the fluxes of every face touching them into LDS, then each
 // cell sums its faces from there.  The faces of block b are list entries [off[b], off[b+1]);
 // a face between two blocks appears in both lists, and the face geometry (Geom face arrays,
 // stride nj) is stored in list order, so each block streams its faces contiguously.
@@ -757,11 +740,11 @@ struct BlockTopo {
     int max_faces;                  // longest list (LDS rows)
 };
 
-template <typename S, int FLUX, int VISC, int ORDER, int MODE>
-__global__ __launch_bounds__(FLUX_THREADS) void k_flux_block(Topo t, BlockTopo bt, Geom<S> g, Phys ph,
-                                                      const S *__restrict__ P,
-                                                      const S *__restrict__ GP,
-                                                      S *__restrict__ Pout, StageArgs a)
+template <typename S, int FLUX, int VISC, int ORDER, int MODE, int BS>
+__global__ __launch_bounds__(BS) void k_flux_block(Topo t, BlockTopo bt, Geom<S> g, Phys ph,
+                                                   const S *__restrict__ P,
+                                                   const S *__restrict__ GP,
+                                                   S *__restrict__ Pout, StageArgs a)
 {
     constexpr int CONV = FLUX != FLUX_NONE;
     using R = FaceRec<S, VISC, CONV>;
@@ -777,10 +760,10 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux_block(Topo t, BlockTopo b
     tf.flr = bt.flr;
     // the next face's cell pair is loaded while this face is computed (its gathers need it)
     int2 lr_next = (threadIdx.x < num) ? bt.flr[beg + threadIdx.x] : int2{0, 0};
-    for (int j = threadIdx.x; j < num; j += FLUX_THREADS) {
+    for (int j = threadIdx.x; j < num; j += BS) {
         const int2 lr = lr_next;
-        if (j + FLUX_THREADS < num) {
-            lr_next = bt.flr[beg + j + FLUX_THREADS];
+        if (j + BS < num) {
+            lr_next = bt.flr[beg + j + BS];
         }
         S fv[5], fc[5];
         face_flux<S, FLUX, VISC, ORDER>(tf, g, ph, P, GP, beg + j, lr, fv, fc, &err);
@@ -800,8 +783,8 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux_block(Topo t, BlockTopo b
         atomicMax(&a.scalars->error, err);
     }
     __syncthreads();
-    const int c = (blk * BLOCK) + threadIdx.x;
-    const bool active = (threadIdx.x < BLOCK) && (c < t.ni);
+    const int c = (blk * BS) + threadIdx.x;
+    const bool active = c < t.ni;
     Acc<S> acc;
     acc.zero();
     if (active) {
@@ -810,12 +793,12 @@ __global__ __launch_bounds__(FLUX_THREADS) void k_flux_block(Topo t, BlockTopo b
             for (int v = 0; v < 5; v++) x[v] = lds[(((part ? R::CONV_OFF : 0) + v) * mf) + j];
         });
     }
-    stage_epilogue<S, MODE, FLUX_THREADS>(t, g, ph, acc, c, active, blk, P, Pout, a);
+    stage_epilogue<S, MODE>(t, g, ph, acc, c, active, P, Pout, a);
 }
 
 // ---------------------------------------------------------------- time step + U0
 // local time step (src/equations/timestep.c:7-30) and, if asked, U0 = prim2cons(P)
-// (advance.c:95-98); dtpart[block] = the block's minimum
+// (advance.c:95-98); dtpart[group] = the minimum of each group of 64 rows
 template <typename S>
 __global__ __launch_bounds__(BLOCK) void k_timestep_u0(Topo t, Geom<S> g, Phys ph,
                                                        const S *__restrict__ P, double *U0,
@@ -837,15 +820,9 @@ __global__ __launch_bounds__(BLOCK) void k_timestep_u0(Topo t, Geom<S> g, Phys p
             for (int v = 0; v < 5; v++) U0[((long)v * t.ni) + c] = u[v];
         }
     }
-    __shared__ double red[BLOCK / 64];
     for (int off = 32; off > 0; off >>= 1) dt = fmin(dt, __shfl_xor(dt, off));
-    if ((threadIdx.x & 63) == 0) {
-        red[threadIdx.x >> 6] = dt;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < BLOCK / 64; w++) dt = fmin(dt, red[w]);
-        dtpart[blockIdx.x] = dt;
+    if ((threadIdx.x & 63) == 0 && c < t.ni) {
+        dtpart[c >> 6] = dt;
     }
 }
 

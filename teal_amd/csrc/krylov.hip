@@ -3,10 +3,11 @@
 //
 // Vectors (U, U0, derivative, Newton residual, increment, the Krylov basis) are FP64
 // [5][num_inner] arrays in device cell order on both precisions; the matvec's residual
-// evaluations run on the configured path (FP64 or the mixed FP32 flux path).  The Newton and
-// GMRES control flow, the Hessenberg matrix, the Givens rotations and the back substitution
-// run on the host in the reference's operation order; every vector operation runs on the
-// device with the reference's per-element expression.  The reference's dot products are
+// evaluations run on the configured path (FP64 or the mixed FP32 flux path).  The Newton loop
+// runs on the host; the GMRES iterations, including the Hessenberg matrix, the Givens
+// rotations, the convergence test and the back substitution, run on the device in the
+// reference's operation order, queued by the host ahead of their results.  Every vector
+// operation has the reference's per-element expression.  The reference's dot products are
 // serial sums.  By default they are deterministic two-level reductions with compensated
 // (TwoSum / TwoProduct) accumulation, which round differently from the reference by at most
 // its own summation error; the finite-difference matvec amplifies such last-bit differences
@@ -16,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -50,8 +52,11 @@ constexpr int MAX_PARTS = 1024;  // blocks of a reduction (partial sums read by 
 // ---------------------------------------------------------------- kernels
 // part[block] = partial dot(a, b)
 __global__ __launch_bounds__(RT) void k_dot(const double *__restrict__ a, const double *__restrict__ b,
-                                            long n, double2 *__restrict__ part)
+                                            long n, double2 *__restrict__ part, const int *skip)
 {
+    if (skip && *skip) {
+        return;
+    }
     double2 acc = make_double2(0.0, 0.0);
     for (long e = (blockIdx.x * (long)RT) + threadIdx.x; e < n; e += (long)gridDim.x * RT) {
         add_product(acc, a[e], b[e]);
@@ -72,19 +77,16 @@ __global__ __launch_bounds__(RT) void k_reduce(const double2 *__restrict__ part,
     }
 }
 
-// out[idx] = sqrt(total), the last entry of a Hessenberg column, and host[0..idx] = out[0..idx]:
-// the column written to host-visible memory (k_reduce with take_sqrt and the column's copy)
+// out[idx] = sqrt(total), the last entry of a Hessenberg column (k_reduce with take_sqrt)
 __global__ __launch_bounds__(RT) void k_reduce_column(const double2 *__restrict__ part, int nb,
-                                                      double *out, int idx, double *host)
+                                                      double *out, int idx, const int *skip)
 {
+    if (*skip) {
+        return;
+    }
     double t = reduce_parts(part, nb);
     if (threadIdx.x == 0) {
-        double r = sqrt(t);
-        out[idx] = r;
-        host[idx] = r;
-    }
-    for (int i = threadIdx.x; i < idx; i += RT) {
-        host[i] = out[i];
+        out[idx] = sqrt(t);
     }
 }
 
@@ -93,8 +95,12 @@ __global__ __launch_bounds__(RT) void k_reduce_column(const double2 *__restrict_
 __global__ __launch_bounds__(RT) void k_mgs(double *__restrict__ w, const double *__restrict__ v_prev,
                                             const double *v_next, long n,
                                             const double2 *__restrict__ part_in, int nb,
-                                            double2 *__restrict__ part_out, double *h, int hidx)
+                                            double2 *__restrict__ part_out, double *h, int hidx,
+                                            const int *skip)
 {
+    if (*skip) {
+        return;
+    }
     // the operands of a thread's first PF elements are loaded before the reduction of
     // part_in, so that their latency and the reduction's overlap (same order of operations)
     constexpr int PF = 8;
@@ -142,8 +148,12 @@ __global__ __launch_bounds__(RT) void k_mgs(double *__restrict__ w, const double
 __global__ __launch_bounds__(RT) void k_serial_dot(const double *__restrict__ a,
                                                    const double *__restrict__ b,
                                                    const int *__restrict__ dev_of_ref, int ni,
-                                                   double *out, int idx, int mode, double fd_scale)
+                                                   double *out, int idx, int mode, double fd_scale,
+                                                   const int *skip)
 {
+    if (skip && *skip) {
+        return;
+    }
     __shared__ double prod[RT * 5];
     double sum = 0.0;
     for (int base = 0; base < ni; base += RT) {
@@ -202,8 +212,11 @@ __global__ __launch_bounds__(RT) void k_serial_residual(const double *__restrict
 
 // w -= h[hidx] * v (gmres.c:96-98) with h from device memory
 __global__ __launch_bounds__(RT) void k_axpy_h(double *__restrict__ w, const double *__restrict__ v,
-                                               const double *h, int hidx, long n)
+                                               const double *h, int hidx, long n, const int *skip)
 {
+    if (*skip) {
+        return;
+    }
     const double hv = h[hidx];
     for (long e = (blockIdx.x * (long)RT) + threadIdx.x; e < n; e += (long)gridDim.x * RT) {
         w[e] = w[e] - (hv * v[e]);
@@ -216,8 +229,11 @@ template <typename S>
 __global__ __launch_bounds__(RT) void k_perturb(const double *__restrict__ U, const double *__restrict__ v,
                                                 int ni, const double2 *__restrict__ part, int nb,
                                                 double fd_scale, double gamma, double *eps_io,
-                                                S *__restrict__ P)
+                                                S *__restrict__ P, const int *skip)
 {
+    if (*skip) {
+        return;
+    }
     // the first cell's U and v are loaded before the reduction of the partials (overlap)
     const int c0 = (blockIdx.x * RT) + threadIdx.x;
     double u0[5], v0[5];
@@ -256,8 +272,11 @@ __global__ __launch_bounds__(RT) void k_matvec_finish(const double *__restrict__
                                                       const double *__restrict__ D1,
                                                       const double *__restrict__ D, double step,
                                                       const double *eps_ptr, double *__restrict__ w,
-                                                      long n)
+                                                      long n, const int *skip)
 {
+    if (*skip) {
+        return;
+    }
     const double eps = *eps_ptr;
     for (long e = (blockIdx.x * (long)RT) + threadIdx.x; e < n; e += (long)gridDim.x * RT) {
         w[e] = v[e] - ((step * (D1[e] - D[e])) / eps);
@@ -271,8 +290,11 @@ __global__ __launch_bounds__(RT) void k_matvec_dot(const double *__restrict__ v,
                                                    const double *__restrict__ D, double step,
                                                    const double *eps_ptr, double *__restrict__ w,
                                                    const double *__restrict__ v0, long n,
-                                                   double2 *__restrict__ part)
+                                                   double2 *__restrict__ part, const int *skip)
 {
+    if (*skip) {
+        return;
+    }
     const double eps = *eps_ptr;
     double2 acc = make_double2(0.0, 0.0);
     for (long e = (blockIdx.x * (long)RT) + threadIdx.x; e < n; e += (long)gridDim.x * RT) {
@@ -301,8 +323,11 @@ __global__ __launch_bounds__(RT) void k_scale(double *__restrict__ out, const do
 // dot(basis[iter+1], basis[iter+1]) as k_dot forms it (the matvec's eps)
 __global__ __launch_bounds__(RT) void k_scale_dot(double *__restrict__ out, const double *__restrict__ x,
                                                   const double *__restrict__ d, long n,
-                                                  double2 *__restrict__ part)
+                                                  double2 *__restrict__ part, const int *skip)
 {
+    if (*skip) {
+        return;
+    }
     const double h = *d;
     double2 acc = make_double2(0.0, 0.0);
     for (long e = (blockIdx.x * (long)RT) + threadIdx.x; e < n; e += (long)gridDim.x * RT) {
@@ -318,16 +343,395 @@ __global__ __launch_bounds__(RT) void k_scale_dot(double *__restrict__ out, cons
     }
 }
 
-// increment = sum over i = iter-1 .. 0 of coef[i] basis[i], in that order (gmres.c:128-140)
+// increment = sum over i = iter-1 .. 0 of coef[i] basis[i], in that order (gmres.c:128-140),
+// with iter the solve's iteration count (device memory)
 __global__ __launch_bounds__(RT) void k_increment(double *__restrict__ inc, const double *__restrict__ V,
-                                                  const double *__restrict__ coef, int iter, long n)
+                                                  const double *__restrict__ coef, const int *iters,
+                                                  long n)
 {
+    const int iter = *iters;
     for (long e = (blockIdx.x * (long)RT) + threadIdx.x; e < n; e += (long)gridDim.x * RT) {
         double x = 0.0;
         for (int i = iter - 1; i >= 0; i--) {
             x = x + (coef[i] * V[((long)i * n) + e]);
         }
         inc[e] = x;
+    }
+}
+
+// GMRES bookkeeping on the device, so that the host need not read each Hessenberg column
+// before it queues the next iteration.  gv holds cosines [dim], sines [dim], the rotated
+// right-hand side [dim + 1] and the rotated Hessenberg matrix [(dim + 1) x dim], row-major;
+// state[0] is 0 while the solve runs and its iteration count once it has stopped; every
+// kernel of a later iteration reads it and does nothing.
+__global__ void k_gmres_init(double *gv, int dim, double norm, int *state)
+{
+    for (int i = threadIdx.x; i < (3 * dim) + 1; i += blockDim.x) {
+        gv[i] = (i == 2 * dim) ? norm : 0.0;  // rhs[0] = |r| (gmres.c:67)
+    }
+    if (threadIdx.x == 0) {
+        state[0] = 0;
+    }
+}
+
+// Column j of the Hessenberg matrix, h[i * hs] for i = 0 .. j+1, through the previous
+// rotations (cs, sn), the new rotation and the update of the right-hand side's entries j and
+// j+1, r[0] and r[1] (gmres.c:102-117), with the reference's operation order.  True when the
+// solve stops after this column: the rotated residual is below tol_norm (gmres.c:118-119) or
+// the basis is full.  One thread.
+__device__ inline bool givens_column(double *h, long hs, double *cs, double *sn, double *r, int j,
+                                     int dim, double tol_norm)
+{
+    for (int i = 0; i < j; i++) {
+        double tmp = (cs[i] * h[i * hs]) + (sn[i] * h[(i + 1) * hs]);
+        h[(i + 1) * hs] = (-sn[i] * h[i * hs]) + (cs[i] * h[(i + 1) * hs]);
+        h[i * hs] = tmp;
+    }
+    double rot = sqrt((h[j * hs] * h[j * hs]) + (h[(j + 1) * hs] * h[(j + 1) * hs]));
+    cs[j] = h[j * hs] / rot;
+    sn[j] = h[(j + 1) * hs] / rot;
+    h[j * hs] = rot;
+    r[1] = -sn[j] * r[0];
+    r[0] = cs[j] * r[0];
+    return fabs(r[1]) < tol_norm || j + 1 == dim;
+}
+
+// the solve stopped after `iters` iterations: state for the device, status for the host
+__device__ inline void gmres_stop(int *state, int *status_host, int iters)
+{
+    state[0] = iters;
+    __atomic_store_n(status_host, iters, __ATOMIC_RELAXED);  // host-coherent memory
+}
+
+// Column j of the Hessenberg matrix (scal[0 .. j+1], from the Gram-Schmidt kernels) rotated
+// into gv and the convergence test.  One thread.
+__global__ void k_givens(const double *__restrict__ scal, double *gv, int j, int dim,
+                         double tol_norm, int *state, int *status_host)
+{
+    if (state[0]) {
+        return;
+    }
+    double *H = gv + (3 * dim) + 1;
+    for (int i = 0; i <= j + 1; i++) {
+        H[((long)i * dim) + j] = scal[i];
+    }
+    if (givens_column(H + j, dim, gv, gv + dim, gv + (2 * dim) + j, j, dim, tol_norm)) {
+        gmres_stop(state, status_host, j + 1);
+    }
+}
+
+// ---------------------------------------------------------------- fused Arnoldi column
+// Data that crosses workgroups inside one launch (the partial sums and the barrier words) is
+// written and read only with device-scope relaxed atomics, which are coherent across the
+// XCDs' L2 caches without a cache write-back or invalidation; a writer waits for its stores to
+// complete (s_waitcnt) before it signals, a reader's loads are issued after it has seen the
+// signal.  No other memory is shared within the launch.
+__device__ inline void store_part(double2 *p, double2 v)
+{
+    __hip_atomic_store(&p->x, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&p->y, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline double2 load_part(const double2 *p)
+{
+    double2 v;
+    v.x = __hip_atomic_load(const_cast<double *>(&p->x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    v.y = __hip_atomic_load(const_cast<double *>(&p->y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return v;
+}
+
+// this wave's outstanding memory operations complete (stores acknowledged at their coherence
+// point) before any later one is issued
+__device__ inline void wait_memory()
+{
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+__device__ inline unsigned sync_load(const unsigned *p)
+{
+    return __hip_atomic_load(const_cast<unsigned *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grid-wide barrier of k_arnoldi, whose workgroups are all resident; thread 0 of each
+// workgroup has stored its partials with store_part before the call.  `sync` holds, 256 bytes
+// apart: eight arrival counters (workgroup b counts on counter b % 8), the counter of completed
+// groups, the generation and a failure flag.  The last arrival of a counter resets it and
+// counts its group; the last group resets that count and advances the generation, which the
+// waiting workgroups poll.  A wait gives up after ARNOLDI_SPIN_LIMIT polls (about a second)
+// and raises the failure flag, which ends every other wait at once and makes later launches
+// return at their start, and err_host (host-coherent memory), so that the kernel ends whatever
+// happens; the host then reports the failure and clears the words.
+constexpr int SYNC_STRIDE = 64;  // unsigned words: 256 bytes
+constexpr int SYNC_GEN = 9 * SYNC_STRIDE;
+constexpr int SYNC_FAILED = 10 * SYNC_STRIDE;
+constexpr int SYNC_WORDS = 11 * SYNC_STRIDE;
+constexpr unsigned ARNOLDI_SPIN_LIMIT = 1u << 20;
+
+__device__ inline void grid_barrier(unsigned *sync, unsigned &gen, int *err_host)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned nb = gridDim.x;
+        const unsigned c = blockIdx.x & 7;
+        const unsigned members = (nb - c + 7) / 8;
+        const unsigned groups = nb < 8 ? nb : 8;
+        wait_memory();  // the partials stored before the arrival
+        bool release = false;
+        if (atomicAdd(&sync[c * SYNC_STRIDE], 1u) == members - 1) {
+            atomicExch(&sync[c * SYNC_STRIDE], 0u);
+            wait_memory();
+            release = atomicAdd(&sync[8 * SYNC_STRIDE], 1u) == groups - 1;
+        }
+        if (release) {
+            atomicExch(&sync[8 * SYNC_STRIDE], 0u);
+            wait_memory();
+            atomicAdd(&sync[SYNC_GEN], 1u);
+        }
+        else {
+            unsigned spins = 0;
+            while (sync_load(&sync[SYNC_GEN]) == gen && sync_load(&sync[SYNC_FAILED]) == 0) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins == ARNOLDI_SPIN_LIMIT) {
+                    atomicExch(&sync[SYNC_FAILED], 1u);
+                    __atomic_store_n(err_host, 1, __ATOMIC_RELAXED);
+                    break;
+                }
+            }
+        }
+        gen += 1;
+        wait_memory();
+    }
+    __syncthreads();
+}
+
+constexpr int ARNOLDI_MAX_DIM = 128;
+// k_arnoldi's workgroups hold ARNOLDI_GROUPS groups of RT threads; group q of workgroup b
+// does the work of block b * ARNOLDI_GROUPS + q of the RT-thread kernels (its elements, its
+// partial sums), so that fewer workgroups meet at each grid barrier
+constexpr int ARNOLDI_GROUPS = 4;
+constexpr int ARNOLDI_THREADS = ARNOLDI_GROUPS * RT;
+
+// block_reduce of each group of RT threads separately: the result of group q in its threads
+__device__ inline double2 group_reduce(double2 v)
+{
+    constexpr int WAVES = RT / 64;
+    __shared__ double2 red[ARNOLDI_GROUPS * WAVES];
+    for (int m = 32; m > 0; m >>= 1) {
+        double2 o;
+        o.x = __shfl_xor(v.x, m, 64);
+        o.y = __shfl_xor(v.y, m, 64);
+        v = combine(v, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    const int w0 = (threadIdx.x / RT) * WAVES;
+    double2 out = red[w0];
+    for (int w = 1; w < WAVES; w++) {
+        out = combine(out, red[w0 + w]);
+    }
+    __syncthreads();
+    return out;
+}
+
+// reduce_parts over the partials of every workgroup of the launch (stored before the last
+// grid barrier), in group 0 (the same loads and combination order as reduce_parts in an
+// RT-thread block), the total in every thread of the workgroup
+__device__ inline double total_of_parts(const double2 *part, int nb)
+{
+    __shared__ double total;
+    constexpr int PER = MAX_PARTS / RT;  // partials per thread: all loads in flight at once
+    double2 acc = make_double2(0.0, 0.0);
+    if (threadIdx.x < RT) {
+        double2 p[PER];
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int b = threadIdx.x + (u * RT);
+            if (b < nb) {
+                p[u] = load_part(part + b);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            if (threadIdx.x + (u * RT) < nb) {
+                acc = combine(acc, p[u]);
+            }
+        }
+    }
+    double2 tot = group_reduce(acc);
+    if (threadIdx.x == 0) {
+        total = tot.x + tot.y;
+    }
+    __syncthreads();
+    const double t = total;
+    __syncthreads();  // total is rewritten by the next call
+    return t;
+}
+
+struct ArnoldiArgs {
+    const double *V;      // basis, (dim + 1) vectors of n
+    const double *R;      // residual of the perturbed state
+    const double *D;      // residual at the base point
+    double step;
+    const double *eps;    // the matvec's eps
+    long n;
+    int nb;               // blocks of the RT-thread kernels: parts_of(n)
+    int j, dim;
+    double2 *part0, *part1;
+    double *gv;
+    double tol_norm;
+    int *state;
+    int *status_host;  // [0]: iteration count of a stopped solve, [1]: a wait timed out
+    unsigned *sync;    // SYNC_WORDS: grid barrier
+};
+
+// One GMRES iteration's column on one launch of ceil(nb / ARNOLDI_GROUPS) workgroups, all
+// resident, with w in registers (EPT elements per thread, the elements a grid-stride loop
+// of the RT-thread kernels gives the thread): the matvec's second half
+// w = v_j - step (R - D) / eps (gmres.c:39-49), modified Gram-Schmidt (gmres.c:92-100),
+// hess[j+1][j] = |w|, the rotations and the convergence test (gmres.c:102-119), and unless the
+// solve stops, basis[j+1] = w / hess[j+1][j] (gmres.c:121-125) with the partials of its norm in
+// part[j % 2] for the next matvec's eps.  Every dot product is formed and reduced exactly as
+// k_matvec_dot, k_mgs, k_reduce_column and k_scale_dot form and reduce it (the same partials,
+// element order and combination order), so the column has the same bits as that kernel
+// sequence.  Every workgroup reduces every dot and rotates the column itself (same bits
+// everywhere), so that it knows whether to write the next basis vector without another
+// barrier; workgroup 0 writes the rotated column.
+template <int EPT>
+__global__ __launch_bounds__(ARNOLDI_THREADS) void k_arnoldi(ArnoldiArgs a)
+{
+    // uniform: no workgroup of this launch changes either before every workgroup has read it
+    if (a.state[0] || sync_load(a.sync + SYNC_FAILED)) {
+        return;
+    }
+    __shared__ double hcol[ARNOLDI_MAX_DIM + 1], cs[ARNOLDI_MAX_DIM], sn[ARNOLDI_MAX_DIM];
+    __shared__ double rhs[2];  // entries j and j+1 of the right-hand side
+    __shared__ int stop;
+    unsigned gen = 0;
+    int *err = a.status_host + 1;
+    const int j = a.j, dim = a.dim;
+    const long n = a.n;
+    const int lane = threadIdx.x % RT;
+    const int blk = (blockIdx.x * ARNOLDI_GROUPS) + (threadIdx.x / RT);  // the RT-thread block
+    const bool writes_part = lane == 0 && blk < a.nb;
+    const long stride = (long)a.nb * RT;
+    const long e0 = (blk * (long)RT) + lane;
+    for (int i = threadIdx.x; i < j; i += ARNOLDI_THREADS) {
+        cs[i] = a.gv[i];
+        sn[i] = a.gv[dim + i];
+    }
+    if (threadIdx.x == 0) {
+        rhs[0] = a.gv[(2 * dim) + j];
+        gen = sync_load(&a.sync[SYNC_GEN]);
+    }
+    double w[EPT], vc[EPT], vn[EPT];
+    const double *vj = a.V + ((long)j * n);
+    const double eps = *a.eps;
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < EPT; k++) {
+        const long e = e0 + (k * stride);
+        if (e < n) {
+            vc[k] = a.V[e];
+            double x = vj[e] - ((a.step * (a.R[e] - a.D[e])) / eps);
+            w[k] = x;
+            add_product(acc, vc[k], x);
+        }
+    }
+    double2 tot = group_reduce(acc);
+    if (writes_part) {
+        store_part(a.part0 + blk, tot);
+    }
+    grid_barrier(a.sync, gen, err);
+    for (int i = 0; i <= j; i++) {
+        const double2 *pin = (i & 1) ? a.part1 : a.part0;
+        double2 *pout = (i & 1) ? a.part0 : a.part1;
+        const double *vnext = a.V + ((long)(i + 1) * n);
+        if (i < j) {  // the next basis vector's loads overlap the reduction
+#pragma unroll
+            for (int k = 0; k < EPT; k++) {
+                const long e = e0 + (k * stride);
+                if (e < n) {
+                    vn[k] = vnext[e];
+                }
+            }
+        }
+        const double hv = total_of_parts(pin, a.nb);
+        if (threadIdx.x == 0) {
+            hcol[i] = hv;
+        }
+        acc = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int k = 0; k < EPT; k++) {
+            const long e = e0 + (k * stride);
+            if (e < n) {
+                double x = w[k] - (hv * vc[k]);
+                w[k] = x;
+                add_product(acc, (i < j) ? vn[k] : x, x);
+                if (i < j) {
+                    vc[k] = vn[k];
+                }
+            }
+        }
+        tot = group_reduce(acc);
+        if (writes_part) {
+            store_part(pout + blk, tot);
+        }
+        grid_barrier(a.sync, gen, err);
+    }
+    const double hn = sqrt(total_of_parts(((j + 1) & 1) ? a.part1 : a.part0, a.nb));
+    if (threadIdx.x == 0) {
+        hcol[j + 1] = hn;
+        rhs[1] = 0.0;
+        stop = givens_column(hcol, 1, cs, sn, rhs, j, dim, a.tol_norm);
+        if (blockIdx.x == 0) {
+            double *H = a.gv + (3 * dim) + 1;
+            for (int i = 0; i <= j + 1; i++) {
+                H[((long)i * dim) + j] = hcol[i];
+            }
+            a.gv[j] = cs[j];
+            a.gv[dim + j] = sn[j];
+            a.gv[(2 * dim) + j] = rhs[0];
+            a.gv[(2 * dim) + j + 1] = rhs[1];
+            if (stop) {
+                gmres_stop(a.state, a.status_host, j + 1);
+            }
+        }
+    }
+    __syncthreads();
+    if (stop) {
+        return;
+    }
+    double *vout = const_cast<double *>(a.V) + ((long)(j + 1) * n);
+    acc = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < EPT; k++) {
+        const long e = e0 + (k * stride);
+        if (e < n) {
+            double y = w[k] / hn;
+            vout[e] = y;
+            add_product(acc, y, y);
+        }
+    }
+    tot = group_reduce(acc);
+    if (writes_part) {
+        ((j & 1) ? a.part1 : a.part0)[blk] = tot;
+    }
+}
+
+// back substitution (gmres.c:127-140) into coef[0 .. iter); one thread
+__global__ void k_back_subst(const double *__restrict__ gv, int dim, const int *state,
+                             double *__restrict__ coef)
+{
+    const int iter = state[0];
+    const double *rhs = gv + (2 * dim), *H = gv + (3 * dim) + 1;
+    for (int i = iter - 1; i >= 0; i--) {
+        double c = rhs[i];
+        for (int k = i + 1; k < iter; k++) {
+            c -= H[((long)i * dim) + k] * coef[k];
+        }
+        coef[i] = c / H[((long)i * dim) + i];
     }
 }
 
@@ -388,9 +792,17 @@ struct Krylov {
     double *coef = nullptr;   // back-substitution coefficients (dim)
     double2 *part[2] = {nullptr, nullptr};
     double *scal = nullptr;   // [0..dim]: Hessenberg column, [dim+1]: eps, [dim+2..]: norms
-    double *h_scal = nullptr; // pinned copy of scal (host-coherent: kernels write columns into it)
-    double *h_scal_dev = nullptr;  // h_scal's device address
+    double *h_scal = nullptr; // pinned host copy of scal (norms read by the Newton loop)
+    double *gv = nullptr;     // GMRES rotations, right-hand side, rotated Hessenberg (k_givens)
+    int *state = nullptr;     // [0]: 0 while a solve runs, its iteration count once stopped
+    int *h_status = nullptr;  // pinned host-coherent: [0] the count of the last solve that
+                              // stopped, [1] a fused column's grid barrier timed out
+    int *h_status_dev = nullptr;  // h_status's device address
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};  // one per iteration in flight
     int *dev_of_ref = nullptr; // serial reductions: device row of each inner cell (reference order)
+    unsigned *sync = nullptr;  // grid barrier of the fused column kernel (k_arnoldi)
+    const void *arnoldi = nullptr;  // the k_arnoldi instance for n, or null: separate kernels
+    int arnoldi_ept = 0;            // its elements per thread
     long newton = 0, krylov = 0;  // iterations of the last implicit step (Newton, GMRES)
 };
 
@@ -401,18 +813,74 @@ void teal_krylov_release(TealGpu *g)
         return;
     }
     Krylov *k = g->krylov;
-    void *ptrs[] = {k->V, k->w, k->U, k->D, k->Dm, k->Rn, k->inc, k->coef, k->part[0], k->part[1],
-                    k->scal, k->dev_of_ref};
+    void *ptrs[] = {k->V,    k->w,       k->U,          k->D,  k->Dm,    k->Rn,
+                    k->inc,  k->coef,    k->part[0],    k->part[1], k->scal, k->dev_of_ref,
+                    k->gv,   k->state,   k->sync};
     for (void *p : ptrs) {
         if (p) {
             (void)hipFree(p);
         }
     }
-    if (k->h_scal) {
-        (void)hipHostFree(k->h_scal);
+    for (void *p : {(void *)k->h_scal, (void *)k->h_status}) {
+        if (p) {
+            (void)hipHostFree(p);
+        }
+    }
+    for (hipEvent_t e : k->ev) {
+        if (e) {
+            (void)hipEventDestroy(e);
+        }
     }
     delete k;
     g->krylov = nullptr;
+}
+
+inline int arnoldi_grid(int nb)
+{
+    return (nb + ARNOLDI_GROUPS - 1) / ARNOLDI_GROUPS;
+}
+
+// The fused column kernel for k->n, when one fits: the elements a thread owns fit its
+// registers (at most 8: up to 5 * 2^18 cells), the dimension fits the kernel's shared arrays,
+// and every workgroup can be resident at once (its grid barrier needs all of them running).
+// It is launched as an ordinary kernel: on its stream the kernels before it have finished when
+// it starts, so only other processes' work could hold compute units back, and then its
+// bounded waits end it with an error.  Otherwise (or with TEAL_GPU_ARNOLDI=0 in the
+// environment) each column takes separate kernels.
+static int arnoldi_pick(Krylov *k)
+{
+    k->arnoldi = nullptr;
+    k->arnoldi_ept = 0;
+    const char *env = getenv("TEAL_GPU_ARNOLDI");
+    if ((env && atoi(env) == 0) || k->dim > ARNOLDI_MAX_DIM) {
+        return 0;
+    }
+    const int nb = parts_of(k->n);
+    const long per_thread = (k->n + ((long)nb * RT) - 1) / ((long)nb * RT);
+    const void *fns[] = {(const void *)k_arnoldi<1>, (const void *)k_arnoldi<2>,
+                         (const void *)k_arnoldi<4>, (const void *)k_arnoldi<8>};
+    constexpr int NFN = sizeof(fns) / sizeof(fns[0]);
+    int which = 0;
+    while (which < NFN && (1L << which) < per_thread) which++;
+    if (which == NFN) {
+        return 0;
+    }
+    const void *fn = fns[which];
+    int dev = 0, cus = 0, per_cu = 0;
+    HIPCALL(hipGetDevice(&dev));
+    HIPCALL(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIPCALL(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, ARNOLDI_THREADS, 0));
+    if ((long)per_cu * cus >= arnoldi_grid(nb)) {
+        k->arnoldi = fn;
+        k->arnoldi_ept = 1 << which;
+    }
+    return 0;
+}
+
+int teal_krylov_column_ept(const TealGpu *g)
+{
+    const Krylov *k = g->krylov;
+    return (k && k->arnoldi && !g->serial_reductions) ? k->arnoldi_ept : 0;
 }
 
 static int krylov_reserve(TealGpu *g, int dim)
@@ -439,11 +907,20 @@ static int krylov_reserve(TealGpu *g, int dim)
     HIPCALL(hipMalloc((void **)&k->part[1], sizeof(double2) * MAX_PARTS));
     HIPCALL(hipMalloc((void **)&k->scal, sizeof(double) * (size_t)(dim + 8)));
     HIPCALL(hipHostMalloc((void **)&k->h_scal, sizeof(double) * (size_t)(dim + 8), hipHostMallocCoherent));
-    HIPCALL(hipHostGetDevicePointer((void **)&k->h_scal_dev, k->h_scal, 0));
+    HIPCALL(hipMalloc((void **)&k->gv, sizeof(double) * (size_t)((3 * dim) + 1 + ((dim + 1) * dim))));
+    HIPCALL(hipMalloc((void **)&k->state, sizeof(int) * 4));
+    HIPCALL(hipMemset(k->state, 0, sizeof(int) * 4));
+    HIPCALL(hipHostMalloc((void **)&k->h_status, sizeof(int) * 4, hipHostMallocCoherent));
+    HIPCALL(hipHostGetDevicePointer((void **)&k->h_status_dev, k->h_status, 0));
+    for (hipEvent_t &e : k->ev) {
+        HIPCALL(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
     HIPCALL(hipMalloc((void **)&k->dev_of_ref, sizeof(int) * (size_t)g->ni));
     HIPCALL(hipMemcpy(k->dev_of_ref, g->cell_inv.data(), sizeof(int) * (size_t)g->ni,
                       hipMemcpyHostToDevice));
-    return 0;
+    HIPCALL(hipMalloc((void **)&k->sync, sizeof(unsigned) * SYNC_WORDS));
+    HIPCALL(hipMemset(k->sync, 0, sizeof(unsigned) * SYNC_WORDS));
+    return arnoldi_pick(k);
 }
 
 namespace {
@@ -454,11 +931,13 @@ void norm_into(TealGpu *g, Krylov *k, const double *a, int pbuf, int idx)
 {
     if (g->serial_reductions) {
         hipLaunchKernelGGL(k_serial_dot, dim3(1), dim3(RT), 0, g->stream, a, a,
-                           (const int *)k->dev_of_ref, g->ni, k->scal, idx, 1, 0.0);
+                           (const int *)k->dev_of_ref, g->ni, k->scal, idx, 1, 0.0,
+                           (const int *)nullptr);
         return;
     }
     const int nb = parts_of(k->n);
-    hipLaunchKernelGGL(k_dot, dim3(nb), dim3(RT), 0, g->stream, a, a, k->n, k->part[pbuf]);
+    hipLaunchKernelGGL(k_dot, dim3(nb), dim3(RT), 0, g->stream, a, a, k->n, k->part[pbuf],
+                       (const int *)nullptr);
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(RT), 0, g->stream, (const double2 *)k->part[pbuf], nb,
                        k->scal, idx, 1);
 }
@@ -484,7 +963,20 @@ int derivative_into(TealGpu *g, DevData<S> &d, Krylov *k, double *out)
     return 0;
 }
 
-// gmres (src/simulation/gmres.c:52-149); the increment lands in k->inc
+// gmres (src/simulation/gmres.c:52-149); the increment lands in k->inc.
+//
+// The host queues iteration after iteration without reading them: each iteration's
+// Hessenberg column is rotated and tested on the device, and once a solve has stopped every
+// Krylov kernel of a later iteration does nothing.  Before it queues the next basis vector's
+// residual evaluation, the host waits for the column `ahead` iterations back and stops when
+// that solve has stopped.  An evaluation queued past the stop writes nothing the caller reads
+// (the perturbed state and R, which the Newton update and the next residual overwrite).  The
+// exact Riemann solver queues nothing ahead: its vacuum flag is sticky, and an evaluation the
+// reference does not make must not raise it.
+//
+// A column is one launch of k_arnoldi when k->arnoldi is set (tree reductions),
+// else k_matvec_dot, one k_mgs per basis vector, k_reduce_column and k_givens (or the serial
+// reductions' kernels): the same bits either way.
 template <typename S>
 int gmres(TealGpu *g, DevData<S> &d, Krylov *k, const double *Dbase, double step, double norm,
           double fd_scale, double tol, int dim)
@@ -493,142 +985,122 @@ int gmres(TealGpu *g, DevData<S> &d, Krylov *k, const double *Dbase, double step
     const int nb = parts_of(n);
     const int cg = cells_grid(g->ni);
     const double gamma = g->cfg.gamma;
+    const int *state = k->state;
+    const bool fused = k->arnoldi && !g->serial_reductions;
+    *(volatile int *)k->h_status = 0;
     if (norm <= 0) {
         HIPCALL(hipMemsetAsync(k->inc, 0, sizeof(double) * (size_t)n, g->stream));
         return 0;
     }
-    std::vector<double> rhs(dim + 1, 0.0), cosine(dim, 0.0), sine(dim, 0.0), coef(dim, 0.0);
-    std::vector<double> hess((size_t)(dim + 1) * dim, 0.0);
-    auto H = [&](int i, int j) -> double & { return hess[((size_t)i * dim) + j]; };
-    rhs[0] = norm;
-    double *eps_dev = k->scal + dim + 1;  // eps of the current matvec
+    hipLaunchKernelGGL(k_gmres_init, dim3(1), dim3(RT), 0, g->stream, k->gv, dim, norm, k->state);
     hipLaunchKernelGGL(k_scale, dim3(nb), dim3(RT), 0, g->stream, k->V, (const double *)k->Rn, -1.0,
                        norm, n);
     const double tol_norm = tol * norm;
+    double *eps_dev = k->scal + dim + 1;  // eps of the current matvec
     // first half of the matvec (gmres.c:12-50) for basis vector j: eps = fd_scale / |v| into
     // scal[dim + 1], the primitive state of U + eps v into d.d_P and its residual into d_R
-    // (tree reductions: the partials of |v|^2 are in part[0] already when `have_parts`)
-    auto perturbed_residual = [&](int j, bool have_parts) -> int {
+    // (tree reductions: the partials of |v|^2 are in `parts` already unless it is null)
+    auto perturbed_residual = [&](int j, const double2 *parts) -> int {
         const double *v = k->V + ((size_t)j * n);
-        const double2 *eps_parts = nullptr;
         if (g->serial_reductions) {
             hipLaunchKernelGGL(k_serial_dot, dim3(1), dim3(RT), 0, g->stream, v, v,
-                               (const int *)k->dev_of_ref, g->ni, k->scal, dim + 1, 2, fd_scale);
+                               (const int *)k->dev_of_ref, g->ni, k->scal, dim + 1, 2, fd_scale, state);
+            parts = nullptr;
         }
-        else {
-            if (!have_parts) {
-                hipLaunchKernelGGL(k_dot, dim3(nb), dim3(RT), 0, g->stream, v, v, n, k->part[0]);
-            }
-            eps_parts = k->part[0];
+        else if (!parts) {
+            hipLaunchKernelGGL(k_dot, dim3(nb), dim3(RT), 0, g->stream, v, v, n, k->part[0], state);
+            parts = k->part[0];
         }
         hipLaunchKernelGGL((k_perturb<S>), dim3(cg), dim3(RT), 0, g->stream, (const double *)k->U,
-                           v, g->ni, eps_parts, nb, fd_scale, gamma, eps_dev, d.d_P);
+                           v, g->ni, parts, nb, fd_scale, gamma, eps_dev, d.d_P, state);
         HIPCALL(hipGetLastError());
         return teal_residual_pass(g, d, 0, 0.0, 0);
     };
     // second half of the matvec, w = v - step (R(cons2prim(U + eps v)) - R(U)) / eps, then
-    // modified Gram-Schmidt (gmres.c:92-100): hess[i][j] lands in scal[i], and h_scal
-    // receives the column
+    // modified Gram-Schmidt (gmres.c:92-100): hess[i][j] lands in scal[i]; then the column's
+    // rotations and the convergence test
     auto column = [&](int j) -> int {
         const double *vj = k->V + ((size_t)j * n);
+        if (fused) {
+            ArnoldiArgs a{k->V,       g->d_R,     Dbase, step,     eps_dev,  n,
+                          nb,         j,          dim,   k->part[0], k->part[1], k->gv,
+                          tol_norm,   k->state,   k->h_status_dev, k->sync};
+            void *args[] = {&a};
+            HIPCALL(hipLaunchKernel(k->arnoldi, dim3(arnoldi_grid(nb)), dim3(ARNOLDI_THREADS), args,
+                                    0, g->stream));
+            return 0;
+        }
         if (g->serial_reductions) {
             hipLaunchKernelGGL(k_matvec_finish, dim3(nb), dim3(RT), 0, g->stream, vj,
-                               (const double *)g->d_R, Dbase, step, (const double *)eps_dev, k->w, n);
+                               (const double *)g->d_R, Dbase, step, (const double *)eps_dev, k->w, n,
+                               state);
             for (int i = 0; i <= j; i++) {
                 const double *vi = k->V + ((size_t)i * n);
                 hipLaunchKernelGGL(k_serial_dot, dim3(1), dim3(RT), 0, g->stream, vi,
                                    (const double *)k->w, (const int *)k->dev_of_ref, g->ni, k->scal, i,
-                                   0, 0.0);
+                                   0, 0.0, state);
                 hipLaunchKernelGGL(k_axpy_h, dim3(nb), dim3(RT), 0, g->stream, k->w, vi,
-                                   (const double *)k->scal, i, n);
+                                   (const double *)k->scal, i, n, state);
             }
             hipLaunchKernelGGL(k_serial_dot, dim3(1), dim3(RT), 0, g->stream, (const double *)k->w,
                                (const double *)k->w, (const int *)k->dev_of_ref, g->ni, k->scal,
-                               j + 1, 1, 0.0);
-            HIPCALL(hipGetLastError());
-            HIPCALL(hipMemcpyAsync(k->h_scal, k->scal, sizeof(double) * (size_t)(j + 2),
-                                   hipMemcpyDeviceToHost, g->stream));
+                               j + 1, 1, 0.0, state);
         }
         else {
             hipLaunchKernelGGL(k_matvec_dot, dim3(nb), dim3(RT), 0, g->stream, vj,
                                (const double *)g->d_R, Dbase, step, (const double *)eps_dev, k->w,
-                               (const double *)k->V, n, k->part[0]);
+                               (const double *)k->V, n, k->part[0], state);
             int cur = 0;
             for (int i = 0; i <= j; i++) {
                 const double *next = (i < j) ? k->V + ((size_t)(i + 1) * n) : nullptr;
                 hipLaunchKernelGGL(k_mgs, dim3(nb), dim3(RT), 0, g->stream, k->w,
                                    (const double *)(k->V + ((size_t)i * n)), next, n,
-                                   (const double2 *)k->part[cur], nb, k->part[1 - cur], k->scal, i);
+                                   (const double2 *)k->part[cur], nb, k->part[1 - cur], k->scal, i,
+                                   state);
                 cur = 1 - cur;
             }
             hipLaunchKernelGGL(k_reduce_column, dim3(1), dim3(RT), 0, g->stream,
-                               (const double2 *)k->part[cur], nb, k->scal, j + 1, k->h_scal_dev);
-            HIPCALL(hipGetLastError());
+                               (const double2 *)k->part[cur], nb, k->scal, j + 1, state);
         }
+        hipLaunchKernelGGL(k_givens, dim3(1), dim3(1), 0, g->stream, (const double *)k->scal, k->gv,
+                           j, dim, tol_norm, k->state, k->h_status_dev);
+        HIPCALL(hipGetLastError());
         return 0;
     };
-    // basis vector j + 1 = w / hess[j + 1][j], read on the device, and its perturbed residual
+    // basis vector j + 1 = w / hess[j + 1][j] (the fused column has formed it, with the
+    // partials of its norm in part[j % 2]) and its perturbed residual
     auto advance = [&](int j) -> int {
+        if (fused) {
+            return perturbed_residual(j + 1, k->part[j % 2]);
+        }
         double2 *parts = g->serial_reductions ? nullptr : k->part[0];
         hipLaunchKernelGGL(k_scale_dot, dim3(nb), dim3(RT), 0, g->stream,
                            k->V + ((size_t)(j + 1) * n), (const double *)k->w,
-                           (const double *)(k->scal + j + 1), n, parts);
-        return perturbed_residual(j + 1, parts != nullptr);
+                           (const double *)(k->scal + j + 1), n, parts, state);
+        return perturbed_residual(j + 1, parts);
     };
-    // The next basis vector and its perturbed residual are queued before the host waits for
-    // the Hessenberg column, so the device works while the host reads it and applies the
-    // rotations.  When the iteration converges that work is discarded: it writes only the next
-    // basis vector, eps, the perturbed state and d_R, which the caller overwrites (the Newton
-    // update rewrites d.d_P, the next residual d_R).  The exact Riemann solver is left out: its
-    // vacuum flag is sticky, and an evaluation the reference does not make must not raise it.
-    const bool ahead = g->cfg.flux != TEAL_FLUX_GODUNOV;
-    if (perturbed_residual(0, false)) return -1;
-    int iter = 0;
-    for (; iter < dim; iter++) {
-        const bool next = iter + 1 < dim;
-        if (column(iter)) return -1;
-        HIPCALL(hipEventRecord(g->ev_wait, g->stream));
-        if (next && ahead && advance(iter)) return -1;
-        if (teal_event_wait(g->ev_wait)) return -1;
-        for (int i = 0; i <= iter + 1; i++) {
-            H(i, iter) = k->h_scal[i];
+    const int ahead = (g->cfg.flux == TEAL_FLUX_GODUNOV) ? 0 : 1;
+    constexpr int NEV = sizeof(k->ev) / sizeof(k->ev[0]);
+    static_assert(NEV > 1, "events of the iterations in flight");
+    auto stopped = [&]() { return *(volatile int *)k->h_status != 0; };  // no wait
+    if (perturbed_residual(0, nullptr)) return -1;
+    for (int j = 0; j < dim; j++) {
+        if (column(j)) return -1;
+        HIPCALL(hipEventRecord(k->ev[j % NEV], g->stream));
+        if (j >= ahead && teal_event_wait(k->ev[(j - ahead) % NEV])) return -1;
+        if (((volatile int *)k->h_status)[1]) {
+            break;  // a fused column's wait timed out: reported after the next synchronization
         }
-        // Givens rotations (gmres.c:102-119)
-        for (int i = 0; i < iter; i++) {
-            double tmp = (cosine[i] * H(i, iter)) + (sine[i] * H(i + 1, iter));
-            H(i + 1, iter) = (-sine[i] * H(i, iter)) + (cosine[i] * H(i + 1, iter));
-            H(i, iter) = tmp;
-        }
-        double rot = sqrt((H(iter, iter) * H(iter, iter)) + (H(iter + 1, iter) * H(iter + 1, iter)));
-        cosine[iter] = H(iter, iter) / rot;
-        sine[iter] = H(iter + 1, iter) / rot;
-        H(iter, iter) = rot;
-        rhs[iter + 1] = -sine[iter] * rhs[iter];
-        rhs[iter] = cosine[iter] * rhs[iter];
-        k->krylov += 1;
-        if (fabs(rhs[iter + 1]) < tol_norm) {
-            iter += 1;
+        if (stopped()) {
             break;
         }
-        if (next && !ahead) {
-            hipLaunchKernelGGL(k_scale, dim3(nb), dim3(RT), 0, g->stream,
-                               k->V + ((size_t)(iter + 1) * n), (const double *)k->w, 1.0,
-                               H(iter + 1, iter), n);
-            if (perturbed_residual(iter + 1, false)) return -1;
-        }
+        if (j + 1 < dim && advance(j)) return -1;
     }
-    // back substitution (gmres.c:127-140)
-    for (int i = iter - 1; i >= 0; i--) {
-        coef[i] = rhs[i];
-        for (int j = i + 1; j < iter; j++) {
-            coef[i] -= H(i, j) * coef[j];
-        }
-        coef[i] /= H(i, i);
-    }
-    HIPCALL(hipMemcpyAsync(k->coef, coef.data(), sizeof(double) * (size_t)dim, hipMemcpyHostToDevice,
-                           g->stream));
+    // back substitution (gmres.c:127-140) and the increment, with the solve's iteration count
+    hipLaunchKernelGGL(k_back_subst, dim3(1), dim3(1), 0, g->stream, (const double *)k->gv, dim,
+                       state, k->coef);
     hipLaunchKernelGGL(k_increment, dim3(nb), dim3(RT), 0, g->stream, k->inc, (const double *)k->V,
-                       (const double *)k->coef, iter, n);
+                       (const double *)k->coef, state, n);
     HIPCALL(hipGetLastError());
     return 0;
 }
@@ -695,6 +1167,13 @@ int implicit_step(TealGpu *g, DevData<S> &dm, double *time, double *residual, do
         hipLaunchKernelGGL(k_newton_residual, dim3(nb), dim3(RT), 0, g->stream, k->Rn,
                            (const double *)k->U, (const double *)g->d_U0, (const double *)k->D, step, n);
         if (two_norms(g, k, k->Rn, k->U, slot) || teal_riemann_check(g)) return -1;
+        k->krylov += *(volatile int *)k->h_status;  // the solve's iteration count
+        if (((volatile int *)k->h_status)[1]) {
+            ((volatile int *)k->h_status)[1] = 0;
+            HIPCALL(hipMemsetAsync(k->sync, 0, sizeof(unsigned) * SYNC_WORDS, g->stream));
+            teal_gpu_set_error("implicit_euler: a GMRES column's grid barrier timed out");
+            return -1;
+        }
         norm = k->h_scal[slot];
         cons_norm = k->h_scal[slot + 1];
         fd_scale = fd_relative * fmax(1, cons_norm) / fmax(1, norm);

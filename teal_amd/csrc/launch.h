@@ -70,55 +70,69 @@ template <typename S>
 using BlockLaunch = void (*)(hipStream_t, size_t, Topo, BlockTopo, Geom<S>, Phys, const S *,
                              const S *, S *, StageArgs);
 
-template <typename S, int FLUX, int VISC, int ORDER, int MODE>
+// BS rows (and threads) per workgroup: the block layout the face lists were built for
+template <typename S, int FLUX, int VISC, int ORDER, int MODE, int BS>
 static void launch_block(hipStream_t s, size_t lds, Topo t, BlockTopo bt, Geom<S> g, Phys ph,
                          const S *P, const S *GP, S *Pout, StageArgs a)
 {
     static size_t granted = 64 * 1024;  // dynamic LDS above 64 KiB has to be requested
     if (lds > granted &&
-        hipFuncSetAttribute((const void *)k_flux_block<S, FLUX, VISC, ORDER, MODE>,
+        hipFuncSetAttribute((const void *)k_flux_block<S, FLUX, VISC, ORDER, MODE, BS>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess) {
         granted = lds;
     }
-    hipLaunchKernelGGL((k_flux_block<S, FLUX, VISC, ORDER, MODE>), dim3(grid_of(t.ni)),
-                       dim3(FLUX_THREADS), lds, s, t, bt, g, ph, P, GP, Pout, a);
+    hipLaunchKernelGGL((k_flux_block<S, FLUX, VISC, ORDER, MODE, BS>),
+                       dim3((unsigned)((t.ni + BS - 1) / BS)), dim3(BS), lds, s, t, bt, g, ph, P,
+                       GP, Pout, a);
 }
 
-template <typename S, int FLUX, int VISC, int ORDER>
+template <typename S, int FLUX, int VISC, int ORDER, int BS>
 static BlockLaunch<S> block_mode(int mode)
 {
     switch (mode) {
-        case 0: return launch_block<S, FLUX, VISC, ORDER, 0>;
-        case 1: return launch_block<S, FLUX, VISC, ORDER, 1>;
-        default: return launch_block<S, FLUX, VISC, ORDER, 2>;
+        case 0: return launch_block<S, FLUX, VISC, ORDER, 0, BS>;
+        case 1: return launch_block<S, FLUX, VISC, ORDER, 1, BS>;
+        default: return launch_block<S, FLUX, VISC, ORDER, 2, BS>;
     }
 }
 
-template <typename S, int FLUX>
+template <typename S, int FLUX, int BS>
 static BlockLaunch<S> block_row(int visc, int order, int mode)
 {
     if (order == 1) {
-        return block_mode<S, FLUX, 0, 1>(mode);
+        return block_mode<S, FLUX, 0, 1, BS>(mode);
     }
-    return visc ? block_mode<S, FLUX, 1, 2>(mode) : block_mode<S, FLUX, 0, 2>(mode);
+    return visc ? block_mode<S, FLUX, 1, 2, BS>(mode) : block_mode<S, FLUX, 0, 2, BS>(mode);
+}
+
+template <typename S, int BS>
+static BlockLaunch<S> block_table_bs(int flux, int visc, int order, int mode)
+{
+    switch (flux) {
+        case FLUX_NONE: return block_row<S, FLUX_NONE, BS>(visc, order, mode);
+        case FLUX_GODUNOV: return block_row<S, FLUX_GODUNOV, BS>(visc, order, mode);
+        case FLUX_ROE: return block_row<S, FLUX_ROE, BS>(visc, order, mode);
+        case FLUX_HLL: return block_row<S, FLUX_HLL, BS>(visc, order, mode);
+        case FLUX_HLLC: return block_row<S, FLUX_HLLC, BS>(visc, order, mode);
+        case FLUX_HLLE: return block_row<S, FLUX_HLLE, BS>(visc, order, mode);
+        case FLUX_AUSMD: return block_row<S, FLUX_AUSMD, BS>(visc, order, mode);
+        case FLUX_AUSMDV: return block_row<S, FLUX_AUSMDV, BS>(visc, order, mode);
+        case FLUX_LXF: return block_row<S, FLUX_LXF, BS>(visc, order, mode);
+        case FLUX_CENTRAL: return block_row<S, FLUX_CENTRAL, BS>(visc, order, mode);
+        default: return nullptr;
+    }
 }
 
 template <typename S>
-static BlockLaunch<S> block_table(int flux, int visc, int order, int mode)
+static BlockLaunch<S> block_table(int flux, int visc, int order, int mode, int bs)
 {
-    switch (flux) {
-        case FLUX_NONE: return block_row<S, FLUX_NONE>(visc, order, mode);
-        case FLUX_GODUNOV: return block_row<S, FLUX_GODUNOV>(visc, order, mode);
-        case FLUX_ROE: return block_row<S, FLUX_ROE>(visc, order, mode);
-        case FLUX_HLL: return block_row<S, FLUX_HLL>(visc, order, mode);
-        case FLUX_HLLC: return block_row<S, FLUX_HLLC>(visc, order, mode);
-        case FLUX_HLLE: return block_row<S, FLUX_HLLE>(visc, order, mode);
-        case FLUX_AUSMD: return block_row<S, FLUX_AUSMD>(visc, order, mode);
-        case FLUX_AUSMDV: return block_row<S, FLUX_AUSMDV>(visc, order, mode);
-        case FLUX_LXF: return block_row<S, FLUX_LXF>(visc, order, mode);
-        case FLUX_CENTRAL: return block_row<S, FLUX_CENTRAL>(visc, order, mode);
-        default: return nullptr;
+    if (bs == BLOCK) {
+        return block_table_bs<S, BLOCK>(flux, visc, order, mode);
     }
+    if (bs == SMALL_BLOCK) {
+        return block_table_bs<S, SMALL_BLOCK>(flux, visc, order, mode);
+    }
+    return nullptr;
 }
 
 }  // namespace fv
@@ -133,53 +147,66 @@ void Kern<S>::boundary(hipStream_t s, int num, int nf, const DevData<S> &d, doub
 }
 
 template <typename S>
-int Kern<S>::flux_block(hipStream_t s, int flux, int visc, int order, int mode, size_t lds,
+int Kern<S>::flux_block(hipStream_t s, int flux, int visc, int order, int mode, int bs, size_t lds,
                         fv::Topo t, fv::BlockTopo bt, fv::Geom<S> g, fv::Phys ph, const S *P,
                         const S *GP, S *Pout, fv::StageArgs a)
 {
-    fv::BlockLaunch<S> fn = fv::block_table<S>(flux, visc, order, mode);
+    fv::BlockLaunch<S> fn = fv::block_table<S>(flux, visc, order, mode, bs);
     if (!fn) {
-        teal_gpu_set_error("no device kernel for flux %d (viscous %d, order %d)", flux, visc, order);
+        teal_gpu_set_error("no device kernel for flux %d (viscous %d, order %d, %d-row blocks)", flux,
+                           visc, order, bs);
         return -1;
     }
     fn(s, lds, t, bt, g, ph, P, GP, Pout, a);
     return 0;
 }
 
-template <typename S, int WK>
+template <typename S, int WK, int GB>
 static void grad_limit_wk(hipStream_t s, int lim, int mode, fv::Topo t, fv::Geom<S> g, const S *P,
                           S *GP)
 {
-    dim3 grid(fv::grid_of(t.ni)), block(fv::BLOCK);
+    dim3 grid((unsigned)((t.ni + GB - 1) / GB)), block(GB);
     if (mode == 0) {
         switch (lim) {
-            case 0: hipLaunchKernelGGL((fv::k_grad_limit<S, 0, 0, WK>), grid, block, 0, s, t, g, P, GP); return;
-            case 1: hipLaunchKernelGGL((fv::k_grad_limit<S, 1, 0, WK>), grid, block, 0, s, t, g, P, GP); return;
-            default: hipLaunchKernelGGL((fv::k_grad_limit<S, 2, 0, WK>), grid, block, 0, s, t, g, P, GP); return;
+            case 0: hipLaunchKernelGGL((fv::k_grad_limit<S, 0, 0, WK, GB>), grid, block, 0, s, t, g, P, GP); return;
+            case 1: hipLaunchKernelGGL((fv::k_grad_limit<S, 1, 0, WK, GB>), grid, block, 0, s, t, g, P, GP); return;
+            default: hipLaunchKernelGGL((fv::k_grad_limit<S, 2, 0, WK, GB>), grid, block, 0, s, t, g, P, GP); return;
         }
     }
     if (mode == 2) {
         switch (lim) {
-            case 0: hipLaunchKernelGGL((fv::k_grad_limit<S, 0, 2, WK>), grid, block, 0, s, t, g, P, GP); return;
-            case 1: hipLaunchKernelGGL((fv::k_grad_limit<S, 1, 2, WK>), grid, block, 0, s, t, g, P, GP); return;
-            default: hipLaunchKernelGGL((fv::k_grad_limit<S, 2, 2, WK>), grid, block, 0, s, t, g, P, GP); return;
+            case 0: hipLaunchKernelGGL((fv::k_grad_limit<S, 0, 2, WK, GB>), grid, block, 0, s, t, g, P, GP); return;
+            case 1: hipLaunchKernelGGL((fv::k_grad_limit<S, 1, 2, WK, GB>), grid, block, 0, s, t, g, P, GP); return;
+            default: hipLaunchKernelGGL((fv::k_grad_limit<S, 2, 2, WK, GB>), grid, block, 0, s, t, g, P, GP); return;
         }
     }
     switch (lim) {
-        case 1: hipLaunchKernelGGL((fv::k_grad_limit<S, 1, 1, WK>), grid, block, 0, s, t, g, P, GP); return;
-        default: hipLaunchKernelGGL((fv::k_grad_limit<S, 2, 1, WK>), grid, block, 0, s, t, g, P, GP); return;
+        case 1: hipLaunchKernelGGL((fv::k_grad_limit<S, 1, 1, WK, GB>), grid, block, 0, s, t, g, P, GP); return;
+        default: hipLaunchKernelGGL((fv::k_grad_limit<S, 2, 1, WK, GB>), grid, block, 0, s, t, g, P, GP); return;
     }
 }
 
+template <typename S, int WK>
+static void grad_limit_gb(hipStream_t s, int gb, int lim, int mode, fv::Topo t, fv::Geom<S> g,
+                          const S *P, S *GP)
+{
+    switch (gb) {
+        case 64: grad_limit_wk<S, WK, 64>(s, lim, mode, t, g, P, GP); return;
+        case 128: grad_limit_wk<S, WK, 128>(s, lim, mode, t, g, P, GP); return;
+        default: grad_limit_wk<S, WK, fv::BLOCK>(s, lim, mode, t, g, P, GP); return;
+    }
+}
+
+// gb: threads per workgroup (64, 128 or BLOCK; autotuned)
 template <typename S>
-void Kern<S>::grad_limit(hipStream_t s, int lim, int mode, fv::Topo t, fv::Geom<S> g, const S *P,
-                         S *GP)
+void Kern<S>::grad_limit(hipStream_t s, int lim, int mode, int gb, fv::Topo t, fv::Geom<S> g,
+                         const S *P, S *GP)
 {
     if (t.wmax <= 4) {
-        grad_limit_wk<S, 4>(s, lim, mode, t, g, P, GP);
+        grad_limit_gb<S, 4>(s, gb, lim, mode, t, g, P, GP);
     }
     else {
-        grad_limit_wk<S, fv::W>(s, lim, mode, t, g, P, GP);
+        grad_limit_gb<S, fv::W>(s, gb, lim, mode, t, g, P, GP);
     }
 }
 

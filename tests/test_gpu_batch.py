@@ -251,8 +251,9 @@ def test_autotune_keeps_state_and_results(precision):
     ref.upload_batch(states)
     tuned.upload_batch(states)
     ms = tuned.autotune(4, 0.5)
-    print(f"  autotune ms/step (fused, fused+graph, split, split+graph): {ms}", flush=True)
+    print(f"  autotune ms/step {dict(zip(teal_amd.AUTOTUNE_SHAPES, ms.round(4)))}", flush=True)
     assert np.isfinite(ms).all() and (ms > 0).all()
+    print(f"  picked: {tuned.info()}", flush=True)
     for a, b in zip(tuned.download_batch(), ref.download_batch()):
         np.testing.assert_array_equal(a, b)
     t_r, r_r = ref.run_lserk_batch(6, np.zeros(len(SIZES)), 0.5)
@@ -301,3 +302,46 @@ def test_batch_refuses_single_mesh_steps():
     with pytest.raises(RuntimeError, match="batch"):
         batch.implicit_euler(0.0, 1.0)
     batch.close()
+
+
+@pytest.mark.parametrize("shape", teal_amd.AUTOTUNE_STAGE_SHAPES)
+def test_every_shape_steps_alike(shape):
+    """Each stage shape (fused 256- or 128-row block layouts, split; eager or graph replay) and
+    each gradient workgroup size gives the default shape's steps bit for bit in FP64: state,
+    time and residual (the residual partial sums are kept per 64 rows, whatever the block),
+    on the batch and on one Navier-Stokes Taylor-Green mesh."""
+    print(f"shape {shape}", flush=True)
+    _, _, states, _, ref = build(SIZES)
+    ref.upload_batch(states)
+    t_r, r_r = ref.run_lserk_batch(5, np.zeros(len(SIZES)), 0.5)
+    want = ref.download_batch()
+    ref.close()
+    for gb in (256, 128, 64):
+        _, _, _, _, b = build(SIZES)
+        b.set_shape(shape, gb)
+        b.upload_batch(states)
+        t_g, r_g = b.run_lserk_batch(5, np.zeros(len(SIZES)), 0.5)
+        np.testing.assert_array_equal(t_g, t_r)
+        np.testing.assert_array_equal(r_g, r_r)
+        for got, w in zip(b.download_batch(), want):
+            np.testing.assert_array_equal(got, w)
+        b.close()
+    from oracle_lib import tgv_state
+    n = (12, 10, 9)
+    P = None
+    out = []
+    for sh, gb in (("fused256", 256), (shape, 128)):
+        m = teal_amd.mesh_box(n, (0, 0, 0), (2 * np.pi,) * 3, (1, 1, 1))
+        g = teal_amd.Gpu(m).configure(flux="hllc", viscous=1, limiter="minmod", navierstokes=1,
+                                      mu=1 / 1600)
+        if P is None:
+            a = g.arrays
+            P = tgv_state(a["center"][:a["num_cells"]])
+            P[a["num_inner"]:] = 0
+        g.set_shape(sh, gb)
+        g.upload(P)
+        t = g.run_lserk(4, 0.0, 0.5)
+        out.append((t, g.download()))
+        g.close()
+    assert out[0][0] == out[1][0]
+    np.testing.assert_array_equal(out[0][1], out[1][1])
