@@ -70,17 +70,20 @@ def byte_model(ni, nc, nf, mixed, viscous=True, vk=False):
             "per_cell_update": (b_step + 3 * (b_grad + b_lim + b_flux)) / ni}
 
 
-def measured_traffic(precision, path=os.path.join(ROOT, "profiles", "hbm_traffic_tgv128.json")):
-    """HBM bytes per launch of the fused stage kernel (RK stages and last stages weighted by
-    their dispatch counts) from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this
-    workload (calibrated against a copy of known size), or None."""
+def measured_traffic(precision, block_rows,
+                     path=os.path.join(ROOT, "profiles", "hbm_traffic_tgv128.json")):
+    """HBM bytes per launch of the fused stage kernel in the block layout of `block_rows` rows
+    (RK stages and last stages weighted by their dispatch counts) from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated against a copy of known
+    size), or None."""
     try:
         with open(path) as f:
             data = json.load(f)["kernels"]
     except (OSError, ValueError, KeyError):
         return None
     typ = "float" if precision == "mixed" else "double"
-    rows = [v for k, v in data.items() if k.startswith(f"k_flux_block<{typ},")]
+    rows = [v for k, v in data.items()
+            if k.startswith(f"k_flux_block<{typ},") and k.endswith(f", {block_rows}>")]
     n = sum(r["dispatches"] for r in rows)
     if not n:
         return None
@@ -724,7 +727,8 @@ def main():
             "shape", "autotune_ms_per_step"]
     timed = [k for k in ("grad", "face", "stage") if k + "_gbs" in mx]
     dom = max(timed, key=lambda k: mx[k + "_ms"])  # dominant kernel
-    traffic = measured_traffic("mixed") if (dom == "stage" and mx["fused"]) else None
+    traffic = (measured_traffic("mixed", mx["layout"]["block_rows"])
+               if (dom == "stage" and mx["fused"]) else None)
     line = {
         "metric": "RK cell-updates/s, TGV 128^3, mixed vs FP64 GPU at stated L2 error; %HBM",
         "value": mx["cell_updates_per_s"],
