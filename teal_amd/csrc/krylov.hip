@@ -720,18 +720,54 @@ __global__ __launch_bounds__(ARNOLDI_THREADS) void k_arnoldi(ArnoldiArgs a)
     }
 }
 
-// back substitution (gmres.c:127-140) into coef[0 .. iter); one thread
-__global__ void k_back_subst(const double *__restrict__ gv, int dim, const int *state,
-                             double *__restrict__ coef)
+// back substitution (gmres.c:127-140) into coef[0 .. iter).  For iter <= BACK_MAX the
+// triangle is staged in shared memory by the whole workgroup, lane k of wave 0 holds coef[k]
+// and forms the products H[i][k] * coef[k] of row i at once, and the subtractions run in the
+// reference's order k = i+1 .. iter-1 (with contraction off, c - H * coef rounds the product
+// first, so the precomputed products give the same bits); larger solves take one thread.
+constexpr int BACK_MAX = 64;
+
+__global__ __launch_bounds__(RT) void k_back_subst(const double *__restrict__ gv, int dim,
+                                                   const int *state, double *__restrict__ coef)
 {
     const int iter = state[0];
     const double *rhs = gv + (2 * dim), *H = gv + (3 * dim) + 1;
+    if (iter > BACK_MAX) {
+        if (threadIdx.x == 0) {
+            for (int i = iter - 1; i >= 0; i--) {
+                double c = rhs[i];
+                for (int k = i + 1; k < iter; k++) {
+                    c -= H[((long)i * dim) + k] * coef[k];
+                }
+                coef[i] = c / H[((long)i * dim) + i];
+            }
+        }
+        return;
+    }
+    __shared__ double tri[BACK_MAX * BACK_MAX];
+    for (int e = threadIdx.x; e < iter * iter; e += RT) {
+        const int i = e / iter, k = e % iter;
+        tri[(i * BACK_MAX) + k] = (k >= i) ? H[((long)i * dim) + k] : 0.0;
+    }
+    __syncthreads();
+    if (threadIdx.x >= 64) {
+        return;
+    }
+    const int lane = threadIdx.x;
+    double mine = 0.0;  // coef[lane] once known
     for (int i = iter - 1; i >= 0; i--) {
+        const double p = (lane > i && lane < iter) ? tri[(i * BACK_MAX) + lane] * mine : 0.0;
         double c = rhs[i];
         for (int k = i + 1; k < iter; k++) {
-            c -= H[((long)i * dim) + k] * coef[k];
+            c -= __shfl(p, k, 64);
         }
-        coef[i] = c / H[((long)i * dim) + i];
+        c /= tri[(i * BACK_MAX) + i];
+        if (lane == i) {
+            mine = c;
+        }
+    }
+    if (lane < iter) {
+        coef[lane] = mine;
     }
 }
 
@@ -1097,7 +1133,7 @@ int gmres(TealGpu *g, DevData<S> &d, Krylov *k, const double *Dbase, double step
         if (j + 1 < dim && advance(j)) return -1;
     }
     // back substitution (gmres.c:127-140) and the increment, with the solve's iteration count
-    hipLaunchKernelGGL(k_back_subst, dim3(1), dim3(1), 0, g->stream, (const double *)k->gv, dim,
+    hipLaunchKernelGGL(k_back_subst, dim3(1), dim3(RT), 0, g->stream, (const double *)k->gv, dim,
                        state, k->coef);
     hipLaunchKernelGGL(k_increment, dim3(nb), dim3(RT), 0, g->stream, k->inc, (const double *)k->V,
                        (const double *)k->coef, state, n);
