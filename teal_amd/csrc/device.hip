@@ -18,6 +18,7 @@
 #include <immintrin.h>
 #include <sched.h>
 
+#include "coherent.h"
 #include "launch.h"
 
 using namespace fv;
@@ -69,7 +70,8 @@ __global__ void k_init_scalars(Scalars *scalars, const double *times, double tim
 // chunk result goes to `chunk`, the workgroup that finishes an instance's chunks last (a
 // ticket per instance) combines the chunk results in chunk order and writes the instance's
 // scalars, then resets the ticket for the next launch.  The order of every sum is fixed by the
-// chunk and group numbering alone, never by which workgroup finishes first.
+// chunk and group numbering alone, never by which workgroup finishes first.  Chunk results
+// pass between workgroups through coherent.h's device-scope atomics.
 //
 // Instance i with groups [g0, g1) keeps its chunk results in rows g0 / chunk_size + i onwards
 // of `chunk`: those rows never overlap the next instance's, so `chunk` needs
@@ -85,23 +87,21 @@ static inline unsigned chunks_of(int n, int chunk_size)
     return (unsigned)std::max(1, (n + chunk_size - 1) / chunk_size);
 }
 
-// true in every thread of the workgroup that arrives last of `n` at `ticket` (then with the
-// other workgroups' writes visible and the ticket reset to 0)
+// true in every thread of the workgroup that arrives last of `n` at `ticket` (the ticket then
+// reset to 0).  The chunk result of this workgroup was stored by wave 0 (st_shared_wg); the
+// last workgroup reads the others' with ld_shared_wg after this returns.
 __device__ inline bool last_workgroup(unsigned *ticket, unsigned n)
 {
     __shared__ int last;
-    __threadfence();  // this workgroup's chunk result visible device-wide before its ticket
     __syncthreads();
     if (threadIdx.x == 0) {
+        wait_memory();  // wave 0's chunk-result stores complete before the ticket
         last = (atomicAdd(ticket, 1u) == n - 1);
         if (last) {
             atomicExch(ticket, 0u);
         }
     }
     __syncthreads();
-    if (last) {
-        __threadfence();  // and the others' results visible to this workgroup
-    }
     return last;
 }
 
@@ -148,11 +148,11 @@ __global__ __launch_bounds__(DT_THREADS) void k_dt_min(Scalars *scalars, const d
     if (nch > 1) {
         double *ch = chunk + chunk_row0(g0, inst, DT_CHUNK);
         if (threadIdx.x == 0) {
-            ch[blockIdx.x] = dt;
+            st_shared_wg(&ch[blockIdx.x], dt);
         }
         if (!last_workgroup(&tickets[inst], nch)) return;
         dt = inf;
-        for (int x = threadIdx.x; x < nch; x += DT_THREADS) dt = fmin(dt, ch[x]);
+        for (int x = threadIdx.x; x < nch; x += DT_THREADS) dt = fmin(dt, ld_shared_wg(&ch[x]));
         __syncthreads();  // block_min's shared array is reused
         dt = block_min(dt);
     }
@@ -179,18 +179,21 @@ static_assert(STEP_END_THREADS % 5 == 0 && (STEP_END_SUMS & (STEP_END_SUMS - 1))
               "k_step_end layout");
 
 // red[v] (v < 5, block shared memory of STEP_END_THREADS doubles) = the sum of variable v over
-// partials[e0 .. e1) (rows of 5) in the fixed order above
+// partials[e0 .. e1) (rows of 5) in the fixed order above; SHARED: partials written by other
+// workgroups of the same launch (read with ld_shared_wg)
+template <bool SHARED = false>
 __device__ inline void sum_partials(const double *partials, long e0, long e1, double *red)
 {
+    auto ld = [&](long e) { return SHARED ? ld_shared_wg(&partials[e]) : partials[e]; };
     const int t = threadIdx.x;
     const int lanes = STEP_END_THREADS / 5;
     double acc[STEP_END_SUMS] = {};
     long e = e0 + t;
     for (; e + ((STEP_END_SUMS - 1) * STEP_END_THREADS) < e1; e += STEP_END_SUMS * STEP_END_THREADS) {
-        for (int u = 0; u < STEP_END_SUMS; u++) acc[u] += partials[e + (u * STEP_END_THREADS)];
+        for (int u = 0; u < STEP_END_SUMS; u++) acc[u] += ld(e + (u * STEP_END_THREADS));
     }
     for (int u = 0; e < e1; e += STEP_END_THREADS, u++) {
-        acc[u] += partials[e];
+        acc[u] += ld(e);
     }
     for (int h = STEP_END_SUMS / 2; h > 0; h >>= 1) {
         for (int u = 0; u < h; u++) acc[u] += acc[u + h];
@@ -225,10 +228,10 @@ __global__ __launch_bounds__(STEP_END_THREADS) void k_step_end(Scalars *scalars,
         // sums of squares is exact
         double *ch = chunk + (chunk_row0(g0, inst, STEP_END_CHUNK) * 5);
         if (threadIdx.x < 5) {
-            ch[(blockIdx.x * 5) + threadIdx.x] = red[threadIdx.x];
+            st_shared_wg(&ch[(blockIdx.x * 5) + threadIdx.x], red[threadIdx.x]);
         }
         if (!last_workgroup(&tickets[inst], nch)) return;
-        sum_partials(ch, 0, (long)nch * 5, red);
+        sum_partials<true>(ch, 0, (long)nch * 5, red);
     }
     if (threadIdx.x == 0) {
         Scalars *sc = &scalars[inst];

@@ -21,6 +21,7 @@
 #include <cstring>
 #include <vector>
 
+#include "coherent.h"
 #include "device.h"
 #include "sums.h"
 
@@ -421,35 +422,22 @@ __global__ void k_givens(const double *__restrict__ scal, double *gv, int j, int
 }
 
 // ---------------------------------------------------------------- fused Arnoldi column
-// Data that crosses workgroups inside one launch (the partial sums and the barrier words) is
-// written and read only with device-scope relaxed atomics, which are coherent across the
-// XCDs' L2 caches without a cache write-back or invalidation; a writer waits for its stores to
-// complete (s_waitcnt) before it signals, a reader's loads are issued after it has seen the
-// signal.  No other memory is shared within the launch.
+// The partial sums and barrier words that cross workgroups inside one launch go through
+// coherent.h's device-scope atomics only.
 __device__ inline void store_part(double2 *p, double2 v)
 {
-    __hip_atomic_store(&p->x, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&p->y, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    st_shared_wg(&p->x, v.x);
+    st_shared_wg(&p->y, v.y);
 }
 
 __device__ inline double2 load_part(const double2 *p)
 {
-    double2 v;
-    v.x = __hip_atomic_load(const_cast<double *>(&p->x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    v.y = __hip_atomic_load(const_cast<double *>(&p->y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return v;
-}
-
-// this wave's outstanding memory operations complete (stores acknowledged at their coherence
-// point) before any later one is issued
-__device__ inline void wait_memory()
-{
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    return make_double2(ld_shared_wg(&p->x), ld_shared_wg(&p->y));
 }
 
 __device__ inline unsigned sync_load(const unsigned *p)
 {
-    return __hip_atomic_load(const_cast<unsigned *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return ld_shared_wg(p);
 }
 
 // Grid-wide barrier of k_arnoldi, whose workgroups are all resident; thread 0 of each
