@@ -16,6 +16,7 @@
 // the reference, for verification on small meshes.  Compiled with -ffp-contract=off.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -445,17 +446,18 @@ __device__ inline unsigned sync_load(const unsigned *p)
 // apart: eight arrival counters (workgroup b counts on counter b % 8), the counter of completed
 // groups, the generation and a failure flag.  The last arrival of a counter resets it and
 // counts its group; the last group resets that count and advances the generation, which the
-// waiting workgroups poll.  A wait gives up after ARNOLDI_SPIN_LIMIT polls (about a second)
-// and raises the failure flag, which ends every other wait at once and makes later launches
-// return at their start, and err_host (host-coherent memory), so that the kernel ends whatever
-// happens; the host then reports the failure and clears the words.
+// waiting workgroups poll.  A wait gives up after spin_limit polls (ARNOLDI_SPIN_LIMIT, about a
+// second, unless a test lowers it) and raises the failure flag, which ends every other wait at
+// once and makes later launches return at their start, and err_host (host-coherent memory), so
+// that the kernel ends whatever happens; the host then reports the failure and clears the
+// words.
 constexpr int SYNC_STRIDE = 64;  // unsigned words: 256 bytes
 constexpr int SYNC_GEN = 9 * SYNC_STRIDE;
 constexpr int SYNC_FAILED = 10 * SYNC_STRIDE;
 constexpr int SYNC_WORDS = 11 * SYNC_STRIDE;
 constexpr unsigned ARNOLDI_SPIN_LIMIT = 1u << 20;
 
-__device__ inline void grid_barrier(unsigned *sync, unsigned &gen, int *err_host)
+__device__ inline void grid_barrier(unsigned *sync, unsigned &gen, int *err_host, unsigned spin_limit)
 {
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -479,7 +481,7 @@ __device__ inline void grid_barrier(unsigned *sync, unsigned &gen, int *err_host
             unsigned spins = 0;
             while (sync_load(&sync[SYNC_GEN]) == gen && sync_load(&sync[SYNC_FAILED]) == 0) {
                 __builtin_amdgcn_s_sleep(2);
-                if (++spins == ARNOLDI_SPIN_LIMIT) {
+                if (++spins >= spin_limit) {
                     atomicExch(&sync[SYNC_FAILED], 1u);
                     __atomic_store_n(err_host, 1, __ATOMIC_RELAXED);
                     break;
@@ -572,6 +574,7 @@ struct ArnoldiArgs {
     int *state;
     int *status_host;  // [0]: iteration count of a stopped solve, [1]: a wait timed out
     unsigned *sync;    // SYNC_WORDS: grid barrier
+    unsigned spin_limit;  // polls before a barrier wait gives up
 };
 
 // One GMRES iteration's column on one launch of ceil(nb / ARNOLDI_GROUPS) workgroups, all
@@ -631,7 +634,7 @@ __global__ __launch_bounds__(ARNOLDI_THREADS) void k_arnoldi(ArnoldiArgs a)
     if (writes_part) {
         store_part(a.part0 + blk, tot);
     }
-    grid_barrier(a.sync, gen, err);
+    grid_barrier(a.sync, gen, err, a.spin_limit);
     for (int i = 0; i <= j; i++) {
         const double2 *pin = (i & 1) ? a.part1 : a.part0;
         double2 *pout = (i & 1) ? a.part0 : a.part1;
@@ -666,7 +669,7 @@ __global__ __launch_bounds__(ARNOLDI_THREADS) void k_arnoldi(ArnoldiArgs a)
         if (writes_part) {
             store_part(pout + blk, tot);
         }
-        grid_barrier(a.sync, gen, err);
+        grid_barrier(a.sync, gen, err, a.spin_limit);
     }
     const double hn = sqrt(total_of_parts(((j + 1) & 1) ? a.part1 : a.part0, a.nb));
     if (threadIdx.x == 0) {
@@ -1011,6 +1014,10 @@ int gmres(TealGpu *g, DevData<S> &d, Krylov *k, const double *Dbase, double step
     const double gamma = g->cfg.gamma;
     const int *state = k->state;
     const bool fused = k->arnoldi && !g->serial_reductions;
+    // TEAL_GPU_ARNOLDI_SPIN lowers the barrier's poll limit, so that a test can drive the
+    // time-out path (every wait still ends)
+    const char *spin_env = getenv("TEAL_GPU_ARNOLDI_SPIN");
+    const unsigned spin_limit = spin_env ? (unsigned)std::max(1, atoi(spin_env)) : ARNOLDI_SPIN_LIMIT;
     *(volatile int *)k->h_status = 0;
     if (norm <= 0) {
         HIPCALL(hipMemsetAsync(k->inc, 0, sizeof(double) * (size_t)n, g->stream));
@@ -1048,7 +1055,7 @@ int gmres(TealGpu *g, DevData<S> &d, Krylov *k, const double *Dbase, double step
         if (fused) {
             ArnoldiArgs a{k->V,       g->d_R,     Dbase, step,     eps_dev,  n,
                           nb,         j,          dim,   k->part[0], k->part[1], k->gv,
-                          tol_norm,   k->state,   k->h_status_dev, k->sync};
+                          tol_norm,   k->state,   k->h_status_dev, k->sync, spin_limit};
             void *args[] = {&a};
             HIPCALL(hipLaunchKernel(k->arnoldi, dim3(arnoldi_grid(nb)), dim3(ARNOLDI_THREADS), args,
                                     0, g->stream));

@@ -87,3 +87,40 @@ def test_fused_column_matches_kernel_sequence(n, precision, ept):
     assert fused == plain
     assert sum(s[3][1] for s in fused) > 3 * 3  # several GMRES iterations per solve
     np.testing.assert_array_equal(P_f[:ni], P_p[:ni])
+
+
+def test_barrier_time_out_is_reported_and_leaves_no_trace():
+    """A lowered poll limit (TEAL_GPU_ARNOLDI_SPIN=1) makes the column kernel's barrier waits
+    give up: every wait still ends, the implicit step reports the time-out, and after the
+    state is uploaded again the same handle steps exactly as a fresh one."""
+    print("arnoldi barrier time-out", flush=True)
+    mesh = channel((64, 32, 32))
+    a = _abi.mesh_arrays(mesh)
+    nc, ni = a["num_cells"], a["num_inner"]
+    y = a["center"][:nc, 1]
+    P0 = np.zeros((nc, 5))
+    P0[:, 0] = 1.0
+    P0[:, 1] = (1 - y ** 2) * (1 + 0.01 * np.random.default_rng(5).standard_normal(nc))
+    P0[:, 4] = 1.0 / (GAMMA * 0.2 ** 2)
+    P0[ni:] = 0
+    clean, P_clean, _ = run(mesh, P0, teal_amd.FP64, True, 2, 200.0)
+    g = teal_amd.Gpu(mesh, teal_amd.FP64)
+    g.configure(space_order=2, flux="hllc", viscous=1, limiter=None, navierstokes=1,
+                gamma=GAMMA, mu=0.01, prandtl=0.72,
+                bcs={"bottom": ("wall", None), "top": ("wall", None)})
+    g.upload(P0)
+    os.environ["TEAL_GPU_ARNOLDI_SPIN"] = "1"
+    try:
+        with pytest.raises(RuntimeError, match="grid barrier timed out"):
+            g.implicit_euler(0.0, 200.0, 0.1, 0.3, 30)
+    finally:
+        os.environ.pop("TEAL_GPU_ARNOLDI_SPIN", None)
+    assert g.info()["column_ept"] == 2
+    g.upload(P0)
+    t, out = 0.0, []
+    for _ in range(2):
+        s, t, res = g.implicit_euler(t, 200.0, 0.1, 0.3, 30)
+        out.append((s, t, tuple(res), tuple(g.krylov_stats().values())))
+    np.testing.assert_array_equal(g.download()[:ni], P_clean[:ni])
+    assert out == clean
+    g.close()
