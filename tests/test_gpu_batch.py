@@ -345,3 +345,35 @@ def test_every_shape_steps_alike(shape):
         g.close()
     assert out[0][0] == out[1][0]
     np.testing.assert_array_equal(out[0][1], out[1][1])
+
+
+def test_instances_of_several_chunks_match_single_handles():
+    """The step reductions cut each instance's groups of 64 rows into chunks (2048 groups for
+    the residual, 4096 for the time step) and combine them in chunk order.  Instances of 1, 2
+    and 3 residual chunks (8,000 to 287,496 cells), the largest not starting on a chunk
+    boundary, step exactly as single-mesh handles: time, residual and state bit for bit."""
+    print("batch of multi-chunk instances", flush=True)
+    sizes = [(20, 20, 20), (66, 66, 66), (54, 53, 52)]
+    meshes = [teal_amd.mesh_box(n, periodic=(0, 0, 0)) for n in sizes]
+    bcs = {ent: (name, ctx) for ent, name, ctx in BCS}
+    states = []
+    for i, m in enumerate(meshes):
+        a = _abi.mesh_arrays(m)
+        states.append(tube_state(a["center"], a["num_inner"], a["num_cells"], 40 + i))
+    batch = teal_amd.Batch(meshes, teal_amd.FP64)
+    batch.configure(space_order=2, flux="hllc", limiter="minmod", bcs=bcs)
+    batch.upload_batch(states)
+    times, res = batch.run_lserk_batch(4, np.zeros(len(sizes)), 0.5)
+    got = batch.download_batch()
+    batch.close()
+    for i, m in enumerate(meshes):
+        single = teal_amd.Gpu(m)
+        single.configure(space_order=2, flux="hllc", limiter="minmod", bcs=bcs)
+        single.upload(states[i])
+        t = 0.0
+        for _ in range(4):
+            _, t, r = single.lserk(t, 0.5)
+        assert times[i] == t
+        np.testing.assert_array_equal(res[i], r)
+        np.testing.assert_array_equal(got[i], single.download())
+        single.close()
