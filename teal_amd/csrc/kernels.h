@@ -504,21 +504,26 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
                 qr[v] = pr[v];
             }
         }
-        S b9[9];
-        if (sizeof(S) == 8 && paired) {
-            for (int i = 0; i < 9; i++) b9[i] = bp[i];
-        }
-        else if (paired) {  // FP32 records keep the normal alone
-            b9[0] = nrm.x;
-            b9[1] = nrm.y;
-            b9[2] = nrm.z;
-            face_tangents(b9);
+        S flx[5];
+        if constexpr (sizeof(S) == 4 && FLUX == FLUX_HLLC) {  // FP32: the flux from the normal alone
+            flux_hllc_normal(flx, ql, qr, gamma, nrm);
         }
         else {
-            for (int i = 0; i < 9; i++) b9[i] = ld_stream(&g.basis[((long)i * nf) + f]);
+            S b9[9];
+            if (sizeof(S) == 8 && paired) {
+                for (int i = 0; i < 9; i++) b9[i] = bp[i];
+            }
+            else if (paired) {  // FP32 records keep the normal alone
+                b9[0] = nrm.x;
+                b9[1] = nrm.y;
+                b9[2] = nrm.z;
+                face_tangents(b9);
+            }
+            else {
+                for (int i = 0; i < 9; i++) b9[i] = ld_stream(&g.basis[((long)i * nf) + f]);
+            }
+            convective<FLUX, S>(flx, ql, qr, gamma, m3(b9), err);
         }
-        S flx[5];
-        convective<FLUX, S>(flx, ql, qr, gamma, m3(b9), err);
         for (int v = 0; v < 5; v++) fc[v] = flx[v] * area;
     }
 }

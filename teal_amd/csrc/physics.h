@@ -508,6 +508,78 @@ template <typename T> TEAL_HD void flux_hllc(T *flux, const T *lp, const T *rp, 
     to_global(flux, f, b);
 }
 
+// flux_hllc without the face frame, for the FP32 path: HLLC is invariant under rotations about
+// the normal n, so the flux rotated back to the global frame needs n alone.  With u_n = u.n,
+// the physical flux of a state is {rho u_n, rho u_n u + p n, (e + p) u_n} and the contact
+// state's momentum is rho* (u + (s* - u_n) n) (tangential velocity kept, normal velocity s*);
+// the signal speeds take the normal components of the same Roe averages.  The FP64 path keeps
+// flux_hllc, whose face-frame arithmetic is the reference's (convective.c:279-346).
+template <typename T>
+struct Cart {
+    T rho, un, p, e;
+    V3<T> u, m;
+};
+
+template <typename T> TEAL_HD Cart<T> to_cart(const T *g, T gamma, V3<T> n)
+{
+    Cart<T> s;
+    s.rho = g[0];
+    s.u = v3(g[1], g[2], g[3]);
+    s.p = g[4];
+    s.un = dot(s.u, n);
+    s.m = mul(s.rho, s.u);
+    s.e = (s.p / (gamma - 1)) + (dot(s.m, s.u) / 2);
+    return s;
+}
+
+template <typename T> TEAL_HD void flux_hllc_normal(T *flux, const T *lp, const T *rp, T gamma, V3<T> n)
+{
+    const Cart<T> l = to_cart(lp, gamma, n);
+    const Cart<T> r = to_cart(rp, gamma, n);
+    T hl = (l.e + l.p) / l.rho;
+    T hr = (r.e + r.p) / r.rho;
+    T sl = tsqrt(l.rho);
+    T sr = tsqrt(r.rho);
+    T w = 1 / (sl + sr);
+    V3<T> u = mul(w, add(mul(sl, l.u), mul(sr, r.u)));
+    T h = w * ((sl * hl) + (sr * hr));
+    T c = tsqrt((gamma - 1) * (h - (dot(u, u) / 2)));
+    T cl = tsqrt(gamma * l.p / l.rho);
+    T cr = tsqrt(gamma * r.p / r.rho);
+    T un = dot(u, n);
+    T ssl = tmin(l.un - cl, un - c);
+    T ssr = tmax(r.un + cr, un + c);
+    const bool super = (ssl >= 0) || (ssr <= 0);
+    T ss = 0, ssk = 0, fk = 0;
+    bool left = ssl >= 0;
+    if (!super) {
+        T fl = l.rho * (ssl - l.un);
+        T fr = r.rho * (ssr - r.un);
+        ss = (r.p - l.p + (fl * l.un) - (fr * r.un)) / (fl - fr);
+        left = (ssl <= 0 && 0 <= ss);
+        ssk = left ? ssl : ssr;
+        fk = left ? fl : fr;
+    }
+    const Cart<T> k = left ? l : r;
+    T mn = k.rho * k.un;
+    T frho = mn;
+    V3<T> fm = add(mul(mn, k.u), mul(k.p, n));
+    T fe = (k.e + k.p) * k.un;
+    if (!super) {
+        T factor = fk / (ssk - ss);
+        V3<T> mstar = mul(factor, add(k.u, mul(ss - k.un, n)));
+        T estar = factor * ((k.e / k.rho) + ((ss - k.un) * (ss + (k.p / fk))));
+        frho += ssk * (factor - k.rho);
+        fm = add(fm, mul(ssk, sub(mstar, k.m)));
+        fe += ssk * (estar - k.e);
+    }
+    flux[0] = frho;
+    flux[1] = fm.x;
+    flux[2] = fm.y;
+    flux[3] = fm.z;
+    flux[4] = fe;
+}
+
 template <typename T> TEAL_HD void flux_hlle(T *flux, const T *lp, const T *rp, T gamma, M3<T> b)
 {
     Local<T> l = to_local(lp, gamma, b);
