@@ -477,7 +477,12 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
             gm[v].z -= a3.z;
         }
         S fl[5];
-        viscous_central<S>(fl, pm, gm, gamma, mu, prl, nrm);
+        if constexpr (sizeof(S) == 4) {
+            viscous_central_normal<S>(fl, pm, gm, gamma, mu, prl, nrm);
+        }
+        else {
+            viscous_central<S>(fl, pm, gm, gamma, mu, prl, nrm);
+        }
         for (int v = 0; v < 5; v++) fv[v] = fl[v] * area;
     }
     if (FLUX != FLUX_NONE) {
@@ -490,12 +495,19 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
                 }
                 return v3(gx[(v * 3) + 0], gx[(v * 3) + 1], gx[(v * 3) + 2]);
             };
-            for (int v = 0; v < 5; v++) ql[v] = pl[v] + dot(limited(gl, v), ol);
+            // FP32: (G psi).o = psi (G.o), the offset product first
+            auto delta = [&](const S *gx, int v, V3<S> o) {
+                if (sizeof(S) == 4 && VISC) {
+                    return gx[15 + v] * dot(v3(gx[(v * 3) + 0], gx[(v * 3) + 1], gx[(v * 3) + 2]), o);
+                }
+                return dot(limited(gx, v), o);
+            };
+            for (int v = 0; v < 5; v++) ql[v] = pl[v] + delta(gl, v, ol);
             if (bnd) {  // raw ghost state (derivative.c:203-211)
                 for (int v = 0; v < 5; v++) qr[v] = pr[v];
             }
             else {
-                for (int v = 0; v < 5; v++) qr[v] = pr[v] + dot(limited(gr, v), orr);
+                for (int v = 0; v < 5; v++) qr[v] = pr[v] + delta(gr, v, orr);
             }
         }
         else {

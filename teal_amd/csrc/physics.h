@@ -787,6 +787,31 @@ TEAL_HD void viscous_central(T *flux, const T *p, const V3<T> *g, T gamma, T mu,
     flux[4] = dot(theta, n);
 }
 
+// viscous_central for the FP32 path, from the same tau with the products the flux needs
+// taken along n first: tau is symmetric, so the energy flux (tau u - q).n = (tau n).u - q.n,
+// and the heat flux enters only as q.n = factor (rho (grad p).n - p (grad rho).n).  The FP64
+// path keeps viscous_central, the reference's arithmetic.
+template <typename T>
+TEAL_HD void viscous_central_normal(T *flux, const T *p, const V3<T> *g, T gamma, T mu, T pr, V3<T> n)
+{
+    M3<T> tau;
+    T div = g[1].x + g[2].y + g[3].z;
+    tau.x.x = 2 * mu * (g[1].x - (div / 3));
+    tau.y.y = 2 * mu * (g[2].y - (div / 3));
+    tau.z.z = 2 * mu * (g[3].z - (div / 3));
+    tau.y.x = tau.x.y = mu * (g[1].y + g[2].x);
+    tau.z.x = tau.x.z = mu * (g[1].z + g[3].x);
+    tau.z.y = tau.y.z = mu * (g[2].z + g[3].y);
+    T factor = -mu * gamma / ((gamma - 1) * pr * sqr(p[0]));
+    T qn = factor * ((p[0] * dot(g[4], n)) - (p[4] * dot(g[0], n)));
+    V3<T> m = mv(tau, n);
+    flux[0] = 0;
+    flux[1] = m.x;
+    flux[2] = m.y;
+    flux[3] = m.z;
+    flux[4] = dot(m, v3(p[1], p[2], p[3])) - qn;
+}
+
 // ---- boundary conditions (src/euler/boundary.c:7-271, navierstokes/boundary.c:6-18) --
 template <typename T> TEAL_HD void bc_to_global(T *g, Local<T> l, T gamma, M3<T> b)
 {
