@@ -441,8 +441,10 @@ __device__ inline unsigned sync_load(const unsigned *p)
     return ld_shared_wg(p);
 }
 
-// Grid-wide barrier of k_arnoldi, whose workgroups are all resident; thread 0 of each
-// workgroup has stored its partials with store_part before the call.  `sync` holds, 256 bytes
+// Grid-wide barrier of k_arnoldi, whose workgroups are all resident; lane 0 of each group of RT
+// threads has stored its partial with store_part before the call.  Every wave first waits for
+// its own stores (s_waitcnt covers only the wave that issues it), then thread 0 arrives for the
+// workgroup.  `sync` holds, 256 bytes
 // apart: eight arrival counters (workgroup b counts on counter b % 8), the counter of completed
 // groups, the generation and a failure flag.  The last arrival of a counter resets it and
 // counts its group; the last group resets that count and advances the generation, which the
@@ -459,13 +461,13 @@ constexpr unsigned ARNOLDI_SPIN_LIMIT = 1u << 20;
 
 __device__ inline void grid_barrier(unsigned *sync, unsigned &gen, int *err_host, unsigned spin_limit)
 {
+    wait_memory();  // this wave's partial stored before the workgroup arrives
     __syncthreads();
     if (threadIdx.x == 0) {
         const unsigned nb = gridDim.x;
         const unsigned c = blockIdx.x & 7;
         const unsigned members = (nb - c + 7) / 8;
         const unsigned groups = nb < 8 ? nb : 8;
-        wait_memory();  // the partials stored before the arrival
         bool release = false;
         if (atomicAdd(&sync[c * SYNC_STRIDE], 1u) == members - 1) {
             atomicExch(&sync[c * SYNC_STRIDE], 0u);
