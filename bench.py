@@ -623,7 +623,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-cores", type=int, default=16,
                     help="most physical host cores for the reference CPU baseline (one pinned "
-                         "process each); 16 is a one-GPU box's CPU share")
+                         "process each); 16 is the host share of one GPU on an 8-GPU MI355X node")
     ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE/"
