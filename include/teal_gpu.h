@@ -190,11 +190,14 @@ int teal_gpu_set_graphs(TealGpu *gpu, int enable);
 //   ms[2], ms[3]  fused stage kernel on blocks of 128 rows (its own face lists); eager, graph
 //   ms[4], ms[5]  face-flux + accumulation kernels; eager, graph replay
 //   ms[6..8]      the chosen stage shape with gradient-kernel workgroups of 256, 128, 64
+//   ms[9..11]     the same with the workgroup's own cell states staged in LDS for the
+//                 neighbour gathers
 // Every shape gives the same results, bit for bit in FP64.
-enum { TEAL_AUTOTUNE_STAGE_SHAPES = 6, TEAL_AUTOTUNE_SHAPES = 9 };
-// Sets a stage shape (the index of ms[0..5] above) and a gradient workgroup size (64, 128 or
-// 256) directly; an error if the shape does not fit the device.
-int teal_gpu_set_shape(TealGpu *gpu, int shape, int grad_block);
+enum { TEAL_AUTOTUNE_STAGE_SHAPES = 6, TEAL_AUTOTUNE_SHAPES = 12 };
+// Sets a stage shape (the index of ms[0..5] above), a gradient workgroup size (64, 128 or 256)
+// and whether the gradient kernel stages states in LDS (0 or 1; by default 1 on meshes of
+// tetrahedra) directly; an error if the shape does not fit the device.
+int teal_gpu_set_shape(TealGpu *gpu, int shape, int grad_block, int grad_stage);
 int teal_gpu_autotune(TealGpu *gpu, int steps, double courant, int time_order, int num_stages,
                       double *ms);
 
@@ -212,9 +215,10 @@ double teal_gpu_timing_ms(TealGpu *gpu, int kernel, int *launches);
 // periodic faces, maximum faces per cell, whether residual passes run the fused stage kernel,
 // and whether it reads the centre form of the face geometry (offsets and corrections
 // recomputed from face and cell centres), and how the implicit solver forms a GMRES column.
-// counts[11]: ni, nc, nf, nfi, nfb, W, fused, centre form, rows per fused block (0: split),
+// counts[12]: ni, nc, nf, nfi, nfb, W, fused, centre form, rows per fused block (0: split),
 // gradient workgroup size, vector elements per thread of the one-launch GMRES column (0:
-// separate kernels per Gram-Schmidt step, or no implicit step yet)
+// separate kernels per Gram-Schmidt step, or no implicit step yet), gradient states staged
+// in LDS
 int teal_gpu_info(TealGpu *gpu, long *counts);
 
 // Precision of the device mirror behind an Equations (no counterpart in the reference,

@@ -27,7 +27,8 @@ REDUCE_TREE, REDUCE_SERIAL = 0, 1
 # chosen one with three gradient-kernel workgroup sizes
 AUTOTUNE_STAGE_SHAPES = ("fused256", "fused256+graph", "fused128", "fused128+graph", "split",
                          "split+graph")
-AUTOTUNE_SHAPES = AUTOTUNE_STAGE_SHAPES + ("grad256", "grad128", "grad64")
+AUTOTUNE_SHAPES = AUTOTUNE_STAGE_SHAPES + ("grad256", "grad128", "grad64", "grad256lds",
+                                           "grad128lds", "grad64lds")
 FLUX = {"none": 0, "godunov": 1, "roe": 2, "hll": 3, "hllc": 4, "hlle": 5, "ausmd": 6,
         "ausmdv": 7, "lxf": 8, "central": 9}
 BC = {"symmetry": 1, "slipwall": 1, "supersonic inflow": 2, "supersonic outflow": 3,
@@ -104,7 +105,7 @@ def lib():
         _sig(l, "teal_gpu_run_lserk_batch", C.c_int, VP, C.c_int, DP, DP, D, C.c_int, C.c_int)
         _sig(l, "teal_gpu_set_graphs", C.c_int, VP, C.c_int)
         _sig(l, "teal_gpu_autotune", C.c_int, VP, C.c_int, D, C.c_int, C.c_int, DP)
-        _sig(l, "teal_gpu_set_shape", C.c_int, VP, C.c_int, C.c_int)
+        _sig(l, "teal_gpu_set_shape", C.c_int, VP, C.c_int, C.c_int, C.c_int)
         _sig(l, "teal_gpu_set_source", C.c_int, VP, C.c_int, C.c_int, DP)
         _sig(l, "teal_gpu_average", C.c_int, VP, C.c_int, DP, DP)
         _sig(l, "teal_gpu_norm", C.c_int, VP, DP, DP, DP)
@@ -224,18 +225,23 @@ class Gpu:
 
     def autotune(self, steps, courant, time_order=2, num_stages=3):
         """Per-step ms of the launch shapes named in AUTOTUNE_SHAPES (NaN: unavailable); the
-        handle keeps the fastest stage shape, then the fastest gradient workgroup size."""
+        handle keeps the fastest stage shape, then the fastest gradient workgroup size and
+        gather (direct or LDS-staged)."""
         ms = np.zeros(len(AUTOTUNE_SHAPES))
         check(self.lib.teal_gpu_autotune(self.h, steps, courant, time_order, num_stages, dp(ms)),
               "autotune")
         return ms
 
-    def set_shape(self, shape, grad_block=256):
-        """Stage shape (an index into AUTOTUNE_STAGE_SHAPES or its name) and gradient
-        workgroup size, set directly."""
+    def set_shape(self, shape, grad_block=256, grad_stage=None):
+        """Stage shape (an index into AUTOTUNE_STAGE_SHAPES or its name), gradient workgroup
+        size and LDS staging of the gradient's gathers (None: keep the handle's), set
+        directly."""
         if isinstance(shape, str):
             shape = AUTOTUNE_STAGE_SHAPES.index(shape)
-        check(self.lib.teal_gpu_set_shape(self.h, shape, grad_block), "set_shape")
+        if grad_stage is None:
+            grad_stage = self.info()["grad_stage"]
+        check(self.lib.teal_gpu_set_shape(self.h, shape, grad_block, int(grad_stage)),
+              "set_shape")
         return self
 
     def set_source(self, kind, entity=0, params=(0.0, 1.0)):
@@ -279,11 +285,11 @@ class Gpu:
         return self
 
     def info(self):
-        c = (C.c_long * 11)()
+        c = (C.c_long * 12)()
         check(self.lib.teal_gpu_info(self.h, c), "info")
         keys = ("num_inner", "num_cells", "num_faces", "num_inner_faces", "off_periodic_faces",
                 "max_cell_faces", "fused", "centre_form", "block_rows", "grad_block",
-                "column_ept")
+                "column_ept", "grad_stage")
         return dict(zip(keys, [int(x) for x in c]))
 
     def close(self):

@@ -81,8 +81,8 @@ struct DevData {
 template <typename S>
 struct Kern {
     static void boundary(hipStream_t s, int num, int nf, const DevData<S> &d, double gamma);
-    static void grad_limit(hipStream_t s, int lim, int mode, int gb, fv::Topo t, fv::Geom<S> g,
-                           const S *P, S *GP);
+    static void grad_limit(hipStream_t s, int lim, int mode, int gb, int stage, fv::Topo t,
+                           fv::Geom<S> g, const S *P, S *GP);
     static int face(hipStream_t s, int flux, int visc, int order, fv::Topo t, fv::Geom<S> g,
                     fv::Phys ph, const S *P, const S *GP, S *F, int *error);
     static void accum(hipStream_t s, int visc, int conv, int mode, fv::Topo t, fv::Geom<S> g,
@@ -152,6 +152,8 @@ struct TealGpu {
     int nblocks = 0;   // blocks of fv::BLOCK rows
     int ngroups = 0;   // groups of 64 rows
     int grad_block = fv::BLOCK;  // threads per workgroup of the gradient kernel (autotuned)
+    int grad_stage = 0;          // gradient kernel stages neighbour states in LDS (autotuned;
+                                 // by default on meshes of tetrahedra)
     hipStream_t stream = nullptr;
     hipEvent_t ev_wait = nullptr;  // host waits on the stream by polling this event
     double *d_stage = nullptr;

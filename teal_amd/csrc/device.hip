@@ -833,6 +833,9 @@ static TealGpu *create_handle(const Mesh *mesh, int precision, int ninst, const 
         g->d64.fused = g->d64.block_lds() > 0 && g->d64.block_lds() <= (size_t)g->lds_max;
         g->d32.fused = g->d32.block_lds() > 0 && g->d32.block_lds() <= (size_t)g->lds_max;
     }
+    // the gradient kernel stages neighbour states in LDS on meshes of tetrahedra, whose
+    // gathers miss the caches; the autotuner may choose otherwise
+    g->grad_stage = (rc == 0 && ((precision == TEAL_FP64) ? g->d64.max_inc : g->d32.max_inc) <= 4);
     g->nblocks = grid_of(g->ni);
     // partial sums per group of 64 rows; a batch pads every instance to whole blocks, so an
     // instance's groups are its blocks' groups
@@ -1153,8 +1156,7 @@ int teal_residual_pass(TealGpu *g, DevData<S> &d, int mode, double alpha, int la
         bool timed = g->timing && g->ev_grad.begin(g->stream);
         // inviscid passes keep only the limited gradient (15 values, in rows of 16)
         Kern<S>::grad_limit(g->stream, g->cfg.limiter, g->cfg.viscous ? 0 : 2, g->grad_block,
-                            g->topo(), d.geom(),
-                            d.d_P, d.d_GP);
+                            g->grad_stage, g->topo(), d.geom(), d.d_P, d.d_GP);
         if (timed) {
             g->ev_grad.end(g->stream);
         }
@@ -1365,7 +1367,8 @@ static int gradient_impl(TealGpu *g, DevData<S> &d, double *primitive, double *g
         HIPCALL(hipMemcpyAsync(g->d_stage, gradient, bytes, hipMemcpyHostToDevice, g->stream));
         Kern<S>::to_device(g->stream, g->d_stage, d.d_GP, g->ni, 15, 15, 20, g->d_cell_perm, 0);
     }
-    Kern<S>::grad_limit(g->stream, lim, mode, g->grad_block, g->topo(), d.geom(), d.d_P, d.d_GP);
+    Kern<S>::grad_limit(g->stream, lim, mode, g->grad_block, g->grad_stage, g->topo(), d.geom(),
+                        d.d_P, d.d_GP);
     HIPCALL(hipGetLastError());
     // rows of 20: gradient (15) then psi (5); the limited gradient is G * psi (limiter.c:61-63)
     std::vector<double> GP((size_t)20 * g->mesh_ni);
@@ -2033,7 +2036,7 @@ static int autotune(TealGpu *g, DevData<S> &d, int steps, double courant, const 
     };
     Snapshot<S> snap;
     if (snap.save(g, d)) return -1;
-    const int graphs0 = g->graphs, cur0 = d.cur, gb0 = g->grad_block;
+    const int graphs0 = g->graphs, cur0 = d.cur, gb0 = g->grad_block, st0 = g->grad_stage;
     hipEvent_t t0, t1;
     HIPCALL(hipEventCreate(&t0));
     HIPCALL(hipEventCreate(&t1));
@@ -2077,20 +2080,26 @@ static int autotune(TealGpu *g, DevData<S> &d, int steps, double courant, const 
         d.fused = best_shape < 4;
         d.cur = (best_shape < 2) ? 0 : (best_shape < 4) ? 1 : cur0;
         g->graphs = best_shape & 1;
-        // gradient kernel workgroup sizes on the chosen stage shape
-        static const int sizes[TEAL_AUTOTUNE_SHAPES - TEAL_AUTOTUNE_STAGE_SHAPES] = {BLOCK, 128, 64};
+        // gradient kernel workgroup sizes, direct or LDS-staged gathers, on the chosen stage
+        // shape
+        static const int sizes[3] = {BLOCK, 128, 64};
+        constexpr int NGRAD = TEAL_AUTOTUNE_SHAPES - TEAL_AUTOTUNE_STAGE_SHAPES;
+        static_assert(NGRAD == 6, "gradient shapes: 3 workgroup sizes x direct / staged");
         double best_gb = INFINITY;
-        int pick = gb0;
-        for (int k = 0; k < TEAL_AUTOTUNE_SHAPES - TEAL_AUTOTUNE_STAGE_SHAPES && rc == 0; k++) {
+        int pick = gb0, pick_st = st0;
+        for (int k = 0; k < NGRAD && rc == 0; k++) {
             double *out = &ms[TEAL_AUTOTUNE_STAGE_SHAPES + k];
             *out = NAN;
-            g->grad_block = sizes[k];
+            g->grad_block = sizes[k % 3];
+            g->grad_stage = k / 3;
             rc = time_steps(out);
             if (rc == 0 && *out < best_gb) {
                 best_gb = *out;
-                pick = sizes[k];
+                pick = sizes[k % 3];
+                pick_st = k / 3;
             }
         }
+        g->grad_stage = (rc == 0) ? pick_st : st0;
         g->grad_block = (rc == 0) ? pick : gb0;
     }
     (void)hipEventDestroy(t0);
@@ -2101,6 +2110,7 @@ static int autotune(TealGpu *g, DevData<S> &d, int steps, double courant, const 
         d.fused = fits(cur0);
         g->graphs = graphs0;
         g->grad_block = gb0;
+        g->grad_stage = st0;
     }
     if (snap.restore(g, d)) return -1;
     return rc;
@@ -2125,7 +2135,7 @@ extern "C" int teal_gpu_autotune(TealGpu *g, int steps, double courant, int time
 // one of the autotuner's stage shapes (0 .. TEAL_AUTOTUNE_STAGE_SHAPES - 1) and a gradient
 // workgroup size, set directly
 template <typename S>
-static int set_shape(TealGpu *g, DevData<S> &d, int shape, int grad_block)
+static int set_shape(TealGpu *g, DevData<S> &d, int shape, int grad_block, int grad_stage)
 {
     const bool fused = shape < 4;
     const int layout = (shape < 2) ? 0 : 1;
@@ -2143,21 +2153,23 @@ static int set_shape(TealGpu *g, DevData<S> &d, int shape, int grad_block)
     d.fused = fused;
     g->graphs = shape & 1;
     g->grad_block = grad_block;
+    g->grad_stage = grad_stage;
     drop_graph(g);
     return 0;
 }
 
-extern "C" int teal_gpu_set_shape(TealGpu *g, int shape, int grad_block)
+extern "C" int teal_gpu_set_shape(TealGpu *g, int shape, int grad_block, int grad_stage)
 {
     if (check_ready(g)) return -1;
     if (shape < 0 || shape >= TEAL_AUTOTUNE_STAGE_SHAPES ||
-        (grad_block != 64 && grad_block != 128 && grad_block != BLOCK)) {
-        teal_gpu_set_error("set_shape: invalid shape (%d) or gradient workgroup (%d)", shape,
-                           grad_block);
+        (grad_block != 64 && grad_block != 128 && grad_block != BLOCK) ||
+        (grad_stage != 0 && grad_stage != 1)) {
+        teal_gpu_set_error("set_shape: invalid shape (%d), gradient workgroup (%d) or staging (%d)",
+                           shape, grad_block, grad_stage);
         return -1;
     }
-    return (g->precision == TEAL_FP64) ? set_shape(g, g->d64, shape, grad_block)
-                                       : set_shape(g, g->d32, shape, grad_block);
+    return (g->precision == TEAL_FP64) ? set_shape(g, g->d64, shape, grad_block, grad_stage)
+                                       : set_shape(g, g->d32, shape, grad_block, grad_stage);
 }
 
 // ------------------------------------------------------------------ timing
@@ -2253,6 +2265,7 @@ extern "C" int teal_gpu_info(TealGpu *g, long *counts)
     counts[7] = fused && centre;
     counts[8] = fused ? rows : 0;
     counts[9] = g->grad_block;
+    counts[11] = g->grad_stage;
     counts[10] = teal_krylov_column_ept(g);
     return 0;
 }

@@ -243,17 +243,29 @@ __global__ __launch_bounds__(BLOCK) void k_boundary(int num, int nf, const int *
 // gradient alone): compute and limit, GP rows [ni][16] = gradient * psi (limiter.c:61-63)
 // and one unused value, so rows stay 16-byte aligned.
 // WK: incidence rows visited (W, or 4 on meshes of tetrahedra, whose rows 4 and 5 are empty).
-template <typename S, int LIM, int MODE, int WK = W, int GB = BLOCK>
+// STAGE: the workgroup's own GB consecutive rows (a Morton-ordered sub-cube) are staged in LDS,
+// and a neighbour inside that range is read from there, the others from global memory: the
+// same values, fewer gathers.  It pays where the gathers miss (tetrahedra) and costs a barrier
+// where they hit (structured hexahedra); the autotuner picks.
+template <typename S, int LIM, int MODE, int WK = W, int GB = BLOCK, int STAGE = 0>
 __global__ __launch_bounds__(GB) void k_grad_limit(Topo t, Geom<S> g, const S *__restrict__ P,
                                                    S *__restrict__ GP)
 {
-    int c = (xcd_block() * GB) + threadIdx.x;
-    if (c >= t.ni) {
+    __shared__ S sp[STAGE ? 5 : 1][STAGE ? GB : 1];
+    const int c0 = xcd_block() * GB;
+    const bool active = c0 + (int)threadIdx.x < t.ni;
+    if (!STAGE && !active) {
         return;
     }
+    // staged: the workgroup's threads past the last inner row work on row c0 (c0 < ni) and
+    // store nothing, so that every thread reaches the barrier
+    const int c = active ? c0 + (int)threadIdx.x : c0;
     const int ni = t.ni;
     S pc[5], mn[5], mx[5];
     load_rec(pc, P, c, 5);
+    if constexpr (STAGE) {
+        for (int v = 0; v < 5; v++) sp[v][threadIdx.x] = pc[v];
+    }
     for (int v = 0; v < 5; v++) mn[v] = mx[v] = pc[v];
     S gr[15];
     if (MODE != 1) {
@@ -270,10 +282,19 @@ __global__ __launch_bounds__(GB) void k_grad_limit(Topo t, Geom<S> g, const S *_
     typedef S S2 __attribute__((ext_vector_type(2)));
     const S2 *wo = reinterpret_cast<const S2 *>(g.w);
     S pn[WK][5], wk[WK][3], ok[WK][3];
+    const int last = min(c0 + GB, ni);  // staged rows [c0, last): inner rows of this workgroup
+    if constexpr (STAGE) {
+        __syncthreads();
+    }
 #pragma unroll
     for (int k = 0; k < WK; k++) {
-        const long n = (nb[k] < 0) ? c : (nb[k] >> 1);
-        load_rec(pn[k], P, n, 5);
+        const int n = (nb[k] < 0) ? c : (nb[k] >> 1);
+        if (STAGE && n >= c0 && n < last) {
+            for (int v = 0; v < 5; v++) pn[k][v] = sp[v][n - c0];
+        }
+        else {
+            load_rec(pn[k], P, n, 5);
+        }
         const S2 a = ld_stream(&wo[((long)((0 * W) + k) * ni) + c]);
         const S2 b = ld_stream(&wo[((long)((1 * W) + k) * ni) + c]);
         wk[k][0] = a.x;
@@ -330,6 +351,9 @@ __global__ __launch_bounds__(GB) void k_grad_limit(Topo t, Geom<S> g, const S *_
                 }
             }
         }
+    }
+    if (!active) {
+        return;
     }
     if (MODE == 2) {
         S *out = &GP[(long)c * 16];

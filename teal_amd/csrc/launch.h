@@ -161,52 +161,61 @@ int Kern<S>::flux_block(hipStream_t s, int flux, int visc, int order, int mode, 
     return 0;
 }
 
-template <typename S, int WK, int GB>
+template <typename S, int WK, int GB, int ST>
 static void grad_limit_wk(hipStream_t s, int lim, int mode, fv::Topo t, fv::Geom<S> g, const S *P,
                           S *GP)
 {
     dim3 grid((unsigned)((t.ni + GB - 1) / GB)), block(GB);
     if (mode == 0) {
         switch (lim) {
-            case 0: hipLaunchKernelGGL((fv::k_grad_limit<S, 0, 0, WK, GB>), grid, block, 0, s, t, g, P, GP); return;
-            case 1: hipLaunchKernelGGL((fv::k_grad_limit<S, 1, 0, WK, GB>), grid, block, 0, s, t, g, P, GP); return;
-            default: hipLaunchKernelGGL((fv::k_grad_limit<S, 2, 0, WK, GB>), grid, block, 0, s, t, g, P, GP); return;
+            case 0: hipLaunchKernelGGL((fv::k_grad_limit<S, 0, 0, WK, GB, ST>), grid, block, 0, s, t, g, P, GP); return;
+            case 1: hipLaunchKernelGGL((fv::k_grad_limit<S, 1, 0, WK, GB, ST>), grid, block, 0, s, t, g, P, GP); return;
+            default: hipLaunchKernelGGL((fv::k_grad_limit<S, 2, 0, WK, GB, ST>), grid, block, 0, s, t, g, P, GP); return;
         }
     }
     if (mode == 2) {
         switch (lim) {
-            case 0: hipLaunchKernelGGL((fv::k_grad_limit<S, 0, 2, WK, GB>), grid, block, 0, s, t, g, P, GP); return;
-            case 1: hipLaunchKernelGGL((fv::k_grad_limit<S, 1, 2, WK, GB>), grid, block, 0, s, t, g, P, GP); return;
-            default: hipLaunchKernelGGL((fv::k_grad_limit<S, 2, 2, WK, GB>), grid, block, 0, s, t, g, P, GP); return;
+            case 0: hipLaunchKernelGGL((fv::k_grad_limit<S, 0, 2, WK, GB, ST>), grid, block, 0, s, t, g, P, GP); return;
+            case 1: hipLaunchKernelGGL((fv::k_grad_limit<S, 1, 2, WK, GB, ST>), grid, block, 0, s, t, g, P, GP); return;
+            default: hipLaunchKernelGGL((fv::k_grad_limit<S, 2, 2, WK, GB, ST>), grid, block, 0, s, t, g, P, GP); return;
         }
     }
     switch (lim) {
-        case 1: hipLaunchKernelGGL((fv::k_grad_limit<S, 1, 1, WK, GB>), grid, block, 0, s, t, g, P, GP); return;
-        default: hipLaunchKernelGGL((fv::k_grad_limit<S, 2, 1, WK, GB>), grid, block, 0, s, t, g, P, GP); return;
+        case 1: hipLaunchKernelGGL((fv::k_grad_limit<S, 1, 1, WK, GB, ST>), grid, block, 0, s, t, g, P, GP); return;
+        default: hipLaunchKernelGGL((fv::k_grad_limit<S, 2, 1, WK, GB, ST>), grid, block, 0, s, t, g, P, GP); return;
     }
 }
 
-template <typename S, int WK>
+template <typename S, int WK, int ST>
 static void grad_limit_gb(hipStream_t s, int gb, int lim, int mode, fv::Topo t, fv::Geom<S> g,
                           const S *P, S *GP)
 {
     switch (gb) {
-        case 64: grad_limit_wk<S, WK, 64>(s, lim, mode, t, g, P, GP); return;
-        case 128: grad_limit_wk<S, WK, 128>(s, lim, mode, t, g, P, GP); return;
-        default: grad_limit_wk<S, WK, fv::BLOCK>(s, lim, mode, t, g, P, GP); return;
+        case 64: grad_limit_wk<S, WK, 64, ST>(s, lim, mode, t, g, P, GP); return;
+        case 128: grad_limit_wk<S, WK, 128, ST>(s, lim, mode, t, g, P, GP); return;
+        default: grad_limit_wk<S, WK, fv::BLOCK, ST>(s, lim, mode, t, g, P, GP); return;
     }
 }
 
-// gb: threads per workgroup (64, 128 or BLOCK; autotuned)
+// gb: threads per workgroup (64, 128 or BLOCK); stage: neighbour states staged in LDS (both
+// autotuned)
 template <typename S>
-void Kern<S>::grad_limit(hipStream_t s, int lim, int mode, int gb, fv::Topo t, fv::Geom<S> g,
-                         const S *P, S *GP)
+void Kern<S>::grad_limit(hipStream_t s, int lim, int mode, int gb, int stage, fv::Topo t,
+                         fv::Geom<S> g, const S *P, S *GP)
 {
     if (t.wmax <= 4) {
-        grad_limit_gb<S, 4>(s, gb, lim, mode, t, g, P, GP);
+        if (stage) {
+            grad_limit_gb<S, 4, 1>(s, gb, lim, mode, t, g, P, GP);
+        }
+        else {
+            grad_limit_gb<S, 4, 0>(s, gb, lim, mode, t, g, P, GP);
+        }
+    }
+    else if (stage) {
+        grad_limit_gb<S, fv::W, 1>(s, gb, lim, mode, t, g, P, GP);
     }
     else {
-        grad_limit_gb<S, fv::W>(s, gb, lim, mode, t, g, P, GP);
+        grad_limit_gb<S, fv::W, 0>(s, gb, lim, mode, t, g, P, GP);
     }
 }
 

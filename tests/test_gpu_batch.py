@@ -307,18 +307,19 @@ def test_batch_refuses_single_mesh_steps():
 @pytest.mark.parametrize("shape", teal_amd.AUTOTUNE_STAGE_SHAPES)
 def test_every_shape_steps_alike(shape):
     """Each stage shape (fused 256- or 128-row block layouts, split; eager or graph replay) and
-    each gradient workgroup size gives the default shape's steps bit for bit in FP64: state,
-    time and residual (the residual partial sums are kept per 64 rows, whatever the block),
-    on the batch and on one Navier-Stokes Taylor-Green mesh."""
+    each gradient workgroup size, with direct or LDS-staged gathers, gives the default shape's
+    steps bit for bit in FP64: state, time and residual (the residual partial sums are kept
+    per 64 rows, whatever the block), on the batch and on one Navier-Stokes Taylor-Green
+    mesh."""
     print(f"shape {shape}", flush=True)
     _, _, states, _, ref = build(SIZES)
     ref.upload_batch(states)
     t_r, r_r = ref.run_lserk_batch(5, np.zeros(len(SIZES)), 0.5)
     want = ref.download_batch()
     ref.close()
-    for gb in (256, 128, 64):
+    for gb, st in ((256, 0), (128, 1), (64, 0), (256, 1)):
         _, _, _, _, b = build(SIZES)
-        b.set_shape(shape, gb)
+        b.set_shape(shape, gb, st)
         b.upload_batch(states)
         t_g, r_g = b.run_lserk_batch(5, np.zeros(len(SIZES)), 0.5)
         np.testing.assert_array_equal(t_g, t_r)
@@ -330,7 +331,7 @@ def test_every_shape_steps_alike(shape):
     n = (12, 10, 9)
     P = None
     out = []
-    for sh, gb in (("fused256", 256), (shape, 128)):
+    for sh, gb, st in (("fused256", 256, 0), (shape, 128, 1)):
         m = teal_amd.mesh_box(n, (0, 0, 0), (2 * np.pi,) * 3, (1, 1, 1))
         g = teal_amd.Gpu(m).configure(flux="hllc", viscous=1, limiter="minmod", navierstokes=1,
                                       mu=1 / 1600)
@@ -338,7 +339,7 @@ def test_every_shape_steps_alike(shape):
             a = g.arrays
             P = tgv_state(a["center"][:a["num_cells"]])
             P[a["num_inner"]:] = 0
-        g.set_shape(sh, gb)
+        g.set_shape(sh, gb, st)
         g.upload(P)
         t = g.run_lserk(4, 0.0, 0.5)
         out.append((t, g.download()))
