@@ -215,10 +215,10 @@ double teal_gpu_timing_ms(TealGpu *gpu, int kernel, int *launches);
 // periodic faces, maximum faces per cell, whether residual passes run the fused stage kernel,
 // and whether it reads the centre form of the face geometry (offsets and corrections
 // recomputed from face and cell centres), and how the implicit solver forms a GMRES column.
-// counts[12]: ni, nc, nf, nfi, nfb, W, fused, centre form, rows per fused block (0: split),
+// counts[13]: ni, nc, nf, nfi, nfb, W, fused, centre form, rows per fused block (0: split),
 // gradient workgroup size, vector elements per thread of the one-launch GMRES column (0:
 // separate kernels per Gram-Schmidt step, or no implicit step yet), gradient states staged
-// in LDS
+// in LDS, FP64 face tangents rebuilt from the normal (4 instead of 7 face-record pairs)
 int teal_gpu_info(TealGpu *gpu, long *counts);
 
 // Precision of the device mirror behind an Equations (no counterpart in the reference,

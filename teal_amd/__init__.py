@@ -285,11 +285,11 @@ class Gpu:
         return self
 
     def info(self):
-        c = (C.c_long * 12)()
+        c = (C.c_long * 13)()
         check(self.lib.teal_gpu_info(self.h, c), "info")
         keys = ("num_inner", "num_cells", "num_faces", "num_inner_faces", "off_periodic_faces",
                 "max_cell_faces", "fused", "centre_form", "block_rows", "grad_block",
-                "column_ept", "grad_stage")
+                "column_ept", "grad_stage", "tangents")
         return dict(zip(keys, [int(x) for x in c]))
 
     def close(self):

@@ -21,12 +21,14 @@ struct BlockSet {
     int rows = 0;  // device rows per block (0: not built)
     int *d_boff = nullptr, *d_bslot = nullptr;
     int2 *d_bflr = nullptr;
-    S *d_bpairs = nullptr;  // face records in block-list order as pairs (FP32 [5][nj][2])
+    S *d_bpairs = nullptr;  // face records in block-list order as pairs (Geom::pairs)
     S *d_barea = nullptr, *d_bbasis = nullptr, *d_boffl = nullptr, *d_boffr = nullptr,
       *d_bcorr = nullptr;  // face geometry in block-list order (stride num_block_faces)
     S *d_bfcen = nullptr;  // centre form: face centres in block-list order
     int *d_bcrow = nullptr;  // centre form: right-cell centre row per list entry
     bool centre_form = false;
+    bool tangents = false;  // FP64 centre form: tangents rebuilt from the normal (Geom::tangents;
+                            // TEAL_GPU_TANGENTS_STORED=1 at creation keeps the stored basis)
     int num_block_faces = 0;
     int max_block_faces = 0;
 };

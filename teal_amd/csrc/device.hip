@@ -615,6 +615,40 @@ static bool centres_reproduce(const std::vector<int2> &flr, const std::vector<in
     return true;
 }
 
+// q of face_tangents_q for every face (basis [9][nf]) when its expressions reproduce each face's
+// stored tangents, or both negated; empty otherwise.  q is read from the stored basis (nqz is
+// the last component of the second tangent, nqy the magnitude of the first tangent's y) and
+// the reconstruction is checked with the device's expressions.
+static std::vector<double> tangent_q(const std::vector<double> &basis, size_t nf)
+{
+    std::vector<double> q(nf);
+    auto b = [&](int k, size_t f) { return basis[(k * nf) + f]; };
+    for (size_t f = 0; f < nf; f++) {
+        const double nx = b(0, f), ny = b(1, f), nz = b(2, f);
+        const double nqz = b(8, f);
+        if (nqz > 0 && -ny / nqz == b(3, f) && nx / nqz == b(4, f) && 0 == b(5, f) &&
+            -nx * nz / nqz == b(6, f) && -ny * nz / nqz == b(7, f)) {
+            q[f] = nqz;
+            continue;
+        }
+        const double nqy = fabs(b(4, f));
+        if (!(nqy > 0)) {
+            return {};
+        }
+        const double t[6] = {-nx * ny / nqy, nqy, -ny * nz / nqy, -nz / nqy, 0, nx / nqy};
+        bool same = true, negated = true;
+        for (int k = 0; k < 6; k++) {
+            same = same && t[k] == b(3 + k, f);
+            negated = negated && t[k] == -b(3 + k, f);
+        }
+        if (!same && !negated) {
+            return {};
+        }
+        q[f] = -nqy;
+    }
+    return q;
+}
+
 // Face lists of the fused stage kernel in the layout of `rows` rows per block: block b owns
 // device cells [b*rows, (b+1)*rows); its list holds every face with an incidence in the block,
 // once, in device face order, and slot[k][c] = (list position << 1) | side for incidence k
@@ -727,13 +761,27 @@ int DevData<S>::build_blocks(BlockSet<S> &bl, int rows, const std::vector<int> &
         }
         return upload_conv(&bl.d_bpairs, pairs);
     }
-    if (centre_form) {  // (area, b0), (b1, b2), ..., (b7, b8), (fcx, fcy), (fcz, 0) per face
-        std::vector<double> pairs(14 * cap, 0.0);
+    if (centre_form) {
+        // (area, n0), (n1, n2), (fcx, fcy), (fcz, q) per face where the tangents can be rebuilt,
+        // else (area, b0), (b1, b2), ..., (b7, b8), (fcx, fcy), (fcz, 0)
+        const std::vector<double> tq = getenv("TEAL_GPU_TANGENTS_STORED")
+                                           ? std::vector<double>()
+                                           : tangent_q(lbasis, (size_t)nj);
+        bl.tangents = nj > 0 && !tq.empty();
+        const int np = bl.tangents ? 4 : 7;
+        std::vector<double> pairs(2 * np * cap, 0.0);
         for (int j = 0; j < nj; j++) {
             double rec[14] = {larea[j]};
-            for (int k = 0; k < 9; k++) rec[1 + k] = lbasis[((size_t)k * nj) + j];
-            for (int k = 0; k < 3; k++) rec[10 + k] = lfcen[((size_t)k * nj) + j];
-            for (int q = 0; q < 7; q++) {
+            if (bl.tangents) {
+                for (int k = 0; k < 3; k++) rec[1 + k] = lbasis[((size_t)k * nj) + j];
+                for (int k = 0; k < 3; k++) rec[4 + k] = lfcen[((size_t)k * nj) + j];
+                rec[7] = tq[j];
+            }
+            else {
+                for (int k = 0; k < 9; k++) rec[1 + k] = lbasis[((size_t)k * nj) + j];
+                for (int k = 0; k < 3; k++) rec[10 + k] = lfcen[((size_t)k * nj) + j];
+            }
+            for (int q = 0; q < np; q++) {
                 pairs[((((size_t)q * cap) + j) * 2)] = rec[2 * q];
                 pairs[((((size_t)q * cap) + j) * 2) + 1] = rec[(2 * q) + 1];
             }
@@ -764,7 +812,7 @@ Geom<S> DevData<S>::block_geom() const
 {
     const BlockSet<S> &b = lay[cur];
     return Geom<S>{d_vol,     d_proj,     d_eps2,    d_w,     b.d_barea,  b.d_bbasis, b.d_boffl,
-                   b.d_boffr, b.d_bcorr,  b.d_bfcen, d_ccen,  b.d_bcrow,  b.d_bpairs};
+                   b.d_boffr, b.d_bcorr,  b.d_bfcen, d_ccen,  b.d_bcrow,  b.d_bpairs, b.tangents};
 }
 
 template <typename S>
@@ -2267,6 +2315,7 @@ extern "C" int teal_gpu_info(TealGpu *g, long *counts)
     counts[9] = g->grad_block;
     counts[11] = g->grad_stage;
     counts[10] = teal_krylov_column_ept(g);
+    counts[12] = g->precision == TEAL_FP64 && fused && g->d64.lay[g->d64.cur].tangents;
     return 0;
 }
 

@@ -90,9 +90,42 @@ struct Geom {
                                   // periodic face, whose flr.y is its source cell)
     // face records as 2-wide pairs; when set, area, basis, offl, offr and fcen are not read.
     // FP32 [5][nf][2]: (area, nx), (ny, nz), (olx, oly), (olz, orx), (ory, orz);
-    // FP64 centre form [7][nf][2]: (area, b0), (b1, b2), ..., (b7, b8), (fcx, fcy), (fcz, 0)
+    // FP64 centre form [7][nf][2]: (area, b0), (b1, b2), ..., (b7, b8), (fcx, fcy), (fcz, 0),
+    // or with `tangents` set [4][nf][2]: (area, n0), (n1, n2), (fcx, fcy), (fcz, q), the
+    // tangents rebuilt from the normal and q (face_tangents_q)
     const S *__restrict__ pairs = nullptr;
+    int tangents = 0;
 };
+
+// FP64 face tangents from the stored normal n and q, with the expressions of
+// src/mesh/generate.c:420-435.  q > 0: the first branch, q = nqz; q < 0: the second, -q =
+// nqy.  The reference builds the tangents before it flips x and y of the basis (437-451), so
+// from the stored, possibly flipped, normal these expressions give the stored tangents, or in
+// the second branch both negated.  Every convective flux is odd in the tangential velocity
+// components (they enter as products with the tangents or squared), and IEEE negation is
+// exact, so both give the same flux bits.  The device layout is built with this form only
+// where a host check found that one of the two holds for every face.
+__device__ inline void face_tangents_q(double *b9, double q)
+{
+    const double nx = b9[0], ny = b9[1], nz = b9[2];
+    if (q > 0) {
+        b9[3] = -ny / q;
+        b9[4] = nx / q;
+        b9[5] = 0;
+        b9[6] = -nx * nz / q;
+        b9[7] = -ny * nz / q;
+        b9[8] = q;
+    }
+    else {
+        const double nqy = -q;
+        b9[3] = -nx * ny / nqy;
+        b9[4] = nqy;
+        b9[5] = -ny * nz / nqy;
+        b9[6] = -nz / nqy;
+        b9[7] = 0;
+        b9[8] = nx / nqy;
+    }
+}
 
 // Face tangents from the unit normal n with the construction of src/mesh/generate.c:420-435
 // (FP32 face records keep only n).  The stored basis may hold both tangents negated (the
@@ -405,17 +438,30 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
         if (g.pairs) {
             typedef double d2 __attribute__((ext_vector_type(2)));
             const d2 *q = reinterpret_cast<const d2 *>(g.pairs);
-            d2 x[7];
-            for (int k = 0; k < 7; k++) x[k] = ld_stream(&q[((long)k * nf) + f]);
-            area = x[0].x;
-            bp[0] = x[0].y;
-            for (int k = 1; k < 5; k++) {
-                bp[(2 * k) - 1] = x[k].x;
-                bp[2 * k] = x[k].y;
+            V3<S> fc;
+            if (g.tangents) {  // (area, n0), (n1, n2), (fcx, fcy), (fcz, q)
+                d2 x[4];
+                for (int k = 0; k < 4; k++) x[k] = ld_stream(&q[((long)k * nf) + f]);
+                area = x[0].x;
+                bp[0] = x[0].y;
+                bp[1] = x[1].x;
+                bp[2] = x[1].y;
+                face_tangents_q(bp, x[3].y);
+                fc = v3(x[2].x, x[2].y, x[3].x);
+            }
+            else {
+                d2 x[7];
+                for (int k = 0; k < 7; k++) x[k] = ld_stream(&q[((long)k * nf) + f]);
+                area = x[0].x;
+                bp[0] = x[0].y;
+                for (int k = 1; k < 5; k++) {
+                    bp[(2 * k) - 1] = x[k].x;
+                    bp[2 * k] = x[k].y;
+                }
+                fc = v3(x[5].x, x[5].y, x[6].x);
             }
             nrm = v3(bp[0], bp[1], bp[2]);
             const long rr = g.crow[f];
-            V3<S> fc = v3(x[5].x, x[5].y, x[6].x);
             V3<S> cl = v3(g.ccen[((long)l * 3) + 0], g.ccen[((long)l * 3) + 1], g.ccen[((long)l * 3) + 2]);
             V3<S> cr = v3(g.ccen[(rr * 3) + 0], g.ccen[(rr * 3) + 1], g.ccen[(rr * 3) + 2]);
             ol = v3(fc.x - cl.x, fc.y - cl.y, fc.z - cl.z);

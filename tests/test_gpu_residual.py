@@ -59,6 +59,28 @@ def test_derivative_bitwise(name, case, kind, path):
     np.testing.assert_array_equal(Pg, Po)
 
 
+@pytest.mark.parametrize("name,case,kind", RESIDUAL_CASES, ids=[c[0] for c in RESIDUAL_CASES])
+def test_rebuilt_tangents_match_stored_basis(name, case, kind, monkeypatch):
+    """FP64 face records that keep the normal alone, the tangents rebuilt on the device with
+    the reference's expressions (possibly both negated), give the residual of the stored basis
+    bit for bit, the exact Riemann solver included.  TEAL_GPU_TANGENTS_STORED=1 when a handle
+    is created keeps the stored basis."""
+    print(f"case {name} (rebuilt tangents)", flush=True)
+    monkeypatch.delenv("TEAL_GPU_SPLIT", raising=False)
+    monkeypatch.delenv("TEAL_GPU_TANGENTS_STORED", raising=False)
+    _, _, rebuilt, _, P = setup_pair(case, kind)
+    monkeypatch.setenv("TEAL_GPU_TANGENTS_STORED", "1")
+    _, _, stored, _, _ = setup_pair(case, kind)
+    info = rebuilt.info()
+    assert info["fused"] and info["centre_form"] and info["tangents"] == 1
+    assert stored.info()["tangents"] == 0
+    R1, P1 = rebuilt.derivative(P.copy())
+    R2, P2 = stored.derivative(P.copy())
+    assert np.abs(R1).max() > 0
+    np.testing.assert_array_equal(R1, R2)
+    np.testing.assert_array_equal(P1, P2)
+
+
 LSERK_CASES = RESIDUAL_CASES[:3] + RESIDUAL_CASES[12:15] + LARGE_CASES
 
 
