@@ -406,7 +406,7 @@ __global__ __launch_bounds__(GB) void k_grad_limit(Topo t, Geom<S> g, const S *_
 template <typename S, int FLUX, int VISC, int ORDER>
 __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph,
                                  const S *__restrict__ P, const S *__restrict__ GP, int f,
-                                 int2 lr, S *fv, S *fc, int *err)
+                                 int2 lr, int crow, S *fv, S *fc, int *err)
 {
     const int nf = t.nf;
     const S gamma = (S)ph.gamma;
@@ -461,7 +461,7 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
                 fc = v3(x[5].x, x[5].y, x[6].x);
             }
             nrm = v3(bp[0], bp[1], bp[2]);
-            const long rr = g.crow[f];
+            const long rr = crow;
             V3<S> cl = v3(g.ccen[((long)l * 3) + 0], g.ccen[((long)l * 3) + 1], g.ccen[((long)l * 3) + 2]);
             V3<S> cr = v3(g.ccen[(rr * 3) + 0], g.ccen[(rr * 3) + 1], g.ccen[(rr * 3) + 2]);
             ol = v3(fc.x - cl.x, fc.y - cl.y, fc.z - cl.z);
@@ -488,7 +488,7 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
         }
     }
     else {
-        const long rr = g.crow[f];
+        const long rr = crow;
         V3<S> fc = v3(ld_stream(&g.fcen[(0 * (long)nf) + f]), ld_stream(&g.fcen[(1 * (long)nf) + f]), ld_stream(&g.fcen[(2 * (long)nf) + f]));
         V3<S> cl = v3(g.ccen[((long)l * 3) + 0], g.ccen[((long)l * 3) + 1], g.ccen[((long)l * 3) + 2]);
         V3<S> cr = v3(g.ccen[(rr * 3) + 0], g.ccen[(rr * 3) + 1], g.ccen[(rr * 3) + 2]);
@@ -624,7 +624,8 @@ __global__ __launch_bounds__(BLOCK) void k_face(Topo t, Geom<S> g, Phys ph, cons
     }
     S fv[5], fc[5];
     int err = 0;
-    face_flux<S, FLUX, VISC, ORDER>(t, g, ph, P, GP, f, t.flr[f], fv, fc, &err);
+    face_flux<S, FLUX, VISC, ORDER>(t, g, ph, P, GP, f, t.flr[f], g.crow ? g.crow[f] : 0, fv, fc,
+                                    &err);
     S *out = &F[(long)f * NF];
     if (R::MERGE) {
         for (int v = 0; v < 5; v++) out[v] = fc[v] - fv[v];
@@ -845,15 +846,21 @@ __global__ __launch_bounds__(BS) void k_flux_block(Topo t, BlockTopo bt, Geom<S>
     Topo tf = t;  // face arrays in list order
     tf.nf = bt.nj;
     tf.flr = bt.flr;
-    // the next face's cell pair is loaded while this face is computed (its gathers need it)
+    // the next face's cell pair (and, in the centre form, the row of its right cell's centre)
+    // is loaded while this face is computed (its gathers need it)
     int2 lr_next = (threadIdx.x < num) ? bt.flr[beg + threadIdx.x] : int2{0, 0};
+    int cr_next = (threadIdx.x < num && g.crow) ? g.crow[beg + threadIdx.x] : 0;
     for (int j = threadIdx.x; j < num; j += BS) {
         const int2 lr = lr_next;
+        const int cr = cr_next;
         if (j + BS < num) {
             lr_next = bt.flr[beg + j + BS];
+            if (g.crow) {
+                cr_next = g.crow[beg + j + BS];
+            }
         }
         S fv[5], fc[5];
-        face_flux<S, FLUX, VISC, ORDER>(tf, g, ph, P, GP, beg + j, lr, fv, fc, &err);
+        face_flux<S, FLUX, VISC, ORDER>(tf, g, ph, P, GP, beg + j, lr, cr, fv, fc, &err);
         if (R::MERGE) {
             for (int v = 0; v < 5; v++) lds[(v * mf) + j] = fc[v] - fv[v];
         }
