@@ -661,6 +661,31 @@ struct StageArgs {
     const double *forcing = nullptr;
 };
 
+// Per-cell inputs of stage_epilogue: the volume, U0 (RK stages) and the projections (last
+// stage).  The FP64 stage kernel loads them before its face loop, where their latency hides
+// behind the face fluxes (its LDS allows 2 waves per SIMD, so the registers are free).
+struct CellIn {
+    double vol = 0, u0[5] = {0, 0, 0, 0, 0}, pj[3] = {0, 0, 0};
+};
+
+template <typename S, int MODE>
+__device__ inline CellIn cell_loads(const Topo &t, const Geom<S> &g, const StageArgs &a, int c,
+                                    bool active)
+{
+    CellIn in;
+    if (active) {
+        const int ni = t.ni;
+        in.vol = (double)ld_stream(&g.vol[c]);
+        if (MODE != 0) {
+            for (int v = 0; v < 5; v++) in.u0[v] = ld_stream(&a.U0[((long)v * ni) + c]);
+        }
+        if (MODE == 2) {
+            for (int k = 0; k < 3; k++) in.pj[k] = (double)ld_stream(&g.proj[((long)k * ni) + c]);
+        }
+    }
+    return in;
+}
+
 // Stage epilogue of inner cell c (active) given its accumulated face sums: finalize
 // R = -R/V (derivative.c:253-258); MODE 0 writes R; MODE 1 writes P_new = cons2prim(U0 +
 // (R alpha) step) (advance.c:103-108); MODE 2 also produces the next step's U0, and the dt
@@ -669,14 +694,15 @@ struct StageArgs {
 // Pin is the state the residual is taken of (the source term's context); it may equal Pout.
 template <typename S, int MODE>
 __device__ inline void stage_epilogue(const Topo &t, const Geom<S> &g, const Phys &ph, const Acc<S> &acc,
-                                      int c, bool active, const S *Pin, S *Pout, const StageArgs &a)
+                                      int c, bool active, const S *Pin, S *Pout, const StageArgs &a,
+                                      const CellIn &in)
 {
     const int ni = t.ni;
     double r[5] = {0, 0, 0, 0, 0};
     double pnew[5] = {0, 0, 0, 0, 0};
     double unew[5] = {0, 0, 0, 0, 0};
     if (active) {
-        double vol = (double)ld_stream(&g.vol[c]);
+        const double vol = in.vol;
         if constexpr (sizeof(S) == 8) {  // the reference's divisions (derivative.c:253-258)
             for (int v = 0; v < 5; v++) r[v] = -acc.get(v) / vol;
         }
@@ -696,7 +722,7 @@ __device__ inline void stage_epilogue(const Topo &t, const Geom<S> &g, const Phy
         }
         else {
             double step = a.scalars[a.grp_inst ? a.grp_inst[c >> 6] : 0].step;
-            for (int v = 0; v < 5; v++) unew[v] = ld_stream(&a.U0[((long)v * ni) + c]) + ((r[v] * a.alpha) * step);
+            for (int v = 0; v < 5; v++) unew[v] = in.u0[v] + ((r[v] * a.alpha) * step);
             if constexpr (sizeof(S) == 8) {
                 cons2prim<double>(pnew, unew, ph.gamma);
             }
@@ -712,9 +738,8 @@ __device__ inline void stage_epilogue(const Topo &t, const Geom<S> &g, const Phy
         if (active) {
             double pst[5];  // the stored state, as the next step's timestep() reads it
             for (int v = 0; v < 5; v++) pst[v] = (double)(S)pnew[v];
-            double vol = (double)ld_stream(&g.vol[c]);
-            V3<double> pj = v3((double)ld_stream(&g.proj[c]), (double)ld_stream(&g.proj[(long)ni + c]),
-                                (double)ld_stream(&g.proj[(2 * (long)ni) + c]));
+            const double vol = in.vol;
+            V3<double> pj = v3(in.pj[0], in.pj[1], in.pj[2]);
             dt = ph.navierstokes ? ns_dt<double>(pst, ph.gamma, ph.mu, ph.prandtl, vol, pj)
                                  : euler_dt<double>(pst, ph.gamma, vol, pj);
             double u0n[5];
@@ -744,18 +769,17 @@ __device__ inline void stage_epilogue(const Topo &t, const Geom<S> &g, const Phy
 
 // Sum of the face contributions of inner cell c in reference face order: viscous
 // (R[left] -= Fv*A, R[right] += Fv*A, derivative.c:123-126) then convective (R[left] += Fc*A,
-// R[right] -= Fc*A, derivative.c:197-200).  `fetch(e, part)` returns the 5 values of the
-// face of incidence e (part 0: viscous, 1: convective).
-template <typename S, int VISC, int CONV, typename Fetch>
-__device__ inline void accumulate(const Topo &t, const int *__restrict__ slots, int c, Acc<S> &acc,
-                                  Fetch fetch)
+// R[right] -= Fc*A, derivative.c:197-200).  `slot(k)` returns incidence k of the cell (< 0:
+// none), `fetch(e, part)` the 5 values of the face of incidence e (part 0: viscous, 1:
+// convective).
+template <typename S, int VISC, int CONV, typename Slot, typename Fetch>
+__device__ inline void accumulate(Acc<S> &acc, Slot slot, Fetch fetch)
 {
-    const int ni = t.ni;
     acc.zero();
     if (VISC) {
 #pragma unroll
         for (int k = 0; k < W; k++) {
-            int e = ld_stream(&slots[((long)k * ni) + c]);
+            int e = slot(k);
             if (e < 0) {
                 continue;
             }
@@ -774,7 +798,7 @@ __device__ inline void accumulate(const Topo &t, const int *__restrict__ slots, 
     if (CONV) {
 #pragma unroll
         for (int k = 0; k < W; k++) {
-            int e = ld_stream(&slots[((long)k * ni) + c]);
+            int e = slot(k);
             if (e < 0) {
                 continue;
             }
@@ -806,12 +830,15 @@ __global__ __launch_bounds__(BLOCK) void k_accum(Topo t, Geom<S> g, Phys ph, con
     Acc<S> acc;
     acc.zero();
     if (active) {
-        accumulate<S, R::AVISC, CONV>(t, t.inc, c, acc, [&](int e, int part, S *x) {
-            const S *rec = &F[((long)(e >> 2) * R::NV) + (part ? R::CONV_OFF : 0)];
-            for (int v = 0; v < 5; v++) x[v] = rec[v];
-        });
+        accumulate<S, R::AVISC, CONV>(
+            acc, [&](int k) { return ld_stream(&t.inc[((long)k * t.ni) + c]); },
+            [&](int e, int part, S *x) {
+                const S *rec = &F[((long)(e >> 2) * R::NV) + (part ? R::CONV_OFF : 0)];
+                for (int v = 0; v < 5; v++) x[v] = rec[v];
+            });
     }
-    stage_epilogue<S, MODE>(t, g, ph, acc, c, active, Pout, Pout, a);
+    stage_epilogue<S, MODE>(t, g, ph, acc, c, active, Pout, Pout, a,
+                            cell_loads<S, MODE>(t, g, a, c, active));
 }
 
 // Per-block face lists of the fused kernel: the workgroup of device cells
@@ -846,6 +873,18 @@ __global__ __launch_bounds__(BS) void k_flux_block(Topo t, BlockTopo bt, Geom<S>
     Topo tf = t;  // face arrays in list order
     tf.nf = bt.nj;
     tf.flr = bt.flr;
+    const int c = (blk * BS) + threadIdx.x;
+    const bool active = c < t.ni;
+    // FP64 with viscous and convective records: the cell's incidence slots and epilogue inputs
+    // are loaded before the face loop (registers to spare at the 2 waves per SIMD its LDS
+    // allows); the other kernels load them after it
+    constexpr bool EARLY = sizeof(S) == 8 && R::NV == 10;
+    CellIn in;
+    int sl[W];
+    if constexpr (EARLY) {
+        in = cell_loads<S, MODE>(t, g, a, c, active);
+        for (int k = 0; k < W; k++) sl[k] = active ? ld_stream(&bt.slot[((long)k * t.ni) + c]) : -1;
+    }
     // the next face's cell pair (and, in the centre form, the row of its right cell's centre)
     // is loaded while this face is computed (its gathers need it)
     int2 lr_next = (threadIdx.x < num) ? bt.flr[beg + threadIdx.x] : int2{0, 0};
@@ -877,17 +916,21 @@ __global__ __launch_bounds__(BS) void k_flux_block(Topo t, BlockTopo bt, Geom<S>
         atomicMax(&a.scalars->error, err);
     }
     __syncthreads();
-    const int c = (blk * BS) + threadIdx.x;
-    const bool active = c < t.ni;
+    if constexpr (!EARLY) {
+        in = cell_loads<S, MODE>(t, g, a, c, active);
+    }
     Acc<S> acc;
     acc.zero();
     if (active) {
-        accumulate<S, R::AVISC, CONV>(t, bt.slot, c, acc, [&](int e, int part, S *x) {
-            const int j = e >> 1;
-            for (int v = 0; v < 5; v++) x[v] = lds[(((part ? R::CONV_OFF : 0) + v) * mf) + j];
-        });
+        accumulate<S, R::AVISC, CONV>(
+            acc,
+            [&](int k) { return EARLY ? sl[k] : ld_stream(&bt.slot[((long)k * t.ni) + c]); },
+            [&](int e, int part, S *x) {
+                const int j = e >> 1;
+                for (int v = 0; v < 5; v++) x[v] = lds[(((part ? R::CONV_OFF : 0) + v) * mf) + j];
+            });
     }
-    stage_epilogue<S, MODE>(t, g, ph, acc, c, active, P, Pout, a);
+    stage_epilogue<S, MODE>(t, g, ph, acc, c, active, P, Pout, a, in);
 }
 
 // ---------------------------------------------------------------- time step + U0
