@@ -16,6 +16,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 
@@ -82,8 +83,10 @@ def measured_traffic(precision, block_rows,
     except (OSError, ValueError, KeyError):
         return None
     typ = "float" if precision == "mixed" else "double"
+    # k_flux_block<type, flux, viscous, order, mode, rows, geometry prefetch>
     rows = [v for k, v in data.items()
-            if k.startswith(f"k_flux_block<{typ},") and k.endswith(f", {block_rows}>")]
+            if k.startswith(f"k_flux_block<{typ},")
+            and re.search(rf", {block_rows}(, \d+)?>$", k)]
     n = sum(r["dispatches"] for r in rows)
     if not n:
         return None

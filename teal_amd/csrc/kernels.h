@@ -403,10 +403,13 @@ __global__ __launch_bounds__(GB) void k_grad_limit(Topo t, Geom<S> g, const S *_
 
 // ---------------------------------------------------------------- face fluxes
 // Viscous (fv) and convective (fc) contribution F*A of face f.
-template <typename S, int FLUX, int VISC, int ORDER>
+// GPRE: FP64 records with rebuilt tangents (g.tangents), their 4 pairs loaded by the caller
+// into `pre` (one face ahead).
+template <typename S, int FLUX, int VISC, int ORDER, bool GPRE = false>
 __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph,
                                  const S *__restrict__ P, const S *__restrict__ GP, int f,
-                                 int2 lr, int crow, S *fv, S *fc, int *err)
+                                 int2 lr, int crow, S *fv, S *fc, int *err,
+                                 const S *pre = nullptr)
 {
     const int nf = t.nf;
     const S gamma = (S)ph.gamma;
@@ -439,9 +442,14 @@ __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph
             typedef double d2 __attribute__((ext_vector_type(2)));
             const d2 *q = reinterpret_cast<const d2 *>(g.pairs);
             V3<S> fc;
-            if (g.tangents) {  // (area, n0), (n1, n2), (fcx, fcy), (fcz, q)
+            if (GPRE || g.tangents) {  // (area, n0), (n1, n2), (fcx, fcy), (fcz, q)
                 d2 x[4];
-                for (int k = 0; k < 4; k++) x[k] = ld_stream(&q[((long)k * nf) + f]);
+                if constexpr (GPRE) {
+                    for (int k = 0; k < 4; k++) x[k] = d2{pre[2 * k], pre[(2 * k) + 1]};
+                }
+                else {
+                    for (int k = 0; k < 4; k++) x[k] = ld_stream(&q[((long)k * nf) + f]);
+                }
                 area = x[0].x;
                 bp[0] = x[0].y;
                 bp[1] = x[1].x;
@@ -855,7 +863,9 @@ struct BlockTopo {
     int max_faces;                  // longest list (LDS rows)
 };
 
-template <typename S, int FLUX, int VISC, int ORDER, int MODE, int BS>
+// GEO (FP64 with viscous and convective records and rebuilt tangents, g.tangents): the next
+// face's geometry records are loaded while this face is computed.
+template <typename S, int FLUX, int VISC, int ORDER, int MODE, int BS, int GEO = 0>
 __global__ __launch_bounds__(BS) void k_flux_block(Topo t, BlockTopo bt, Geom<S> g, Phys ph,
                                                    const S *__restrict__ P,
                                                    const S *__restrict__ GP,
@@ -886,20 +896,41 @@ __global__ __launch_bounds__(BS) void k_flux_block(Topo t, BlockTopo bt, Geom<S>
         for (int k = 0; k < W; k++) sl[k] = active ? ld_stream(&bt.slot[((long)k * t.ni) + c]) : -1;
     }
     // the next face's cell pair (and, in the centre form, the row of its right cell's centre)
-    // is loaded while this face is computed (its gathers need it)
+    // is loaded while this face is computed (its gathers need it); with GEO, so are its
+    // geometry records (streamed from HBM, the longest wait)
     int2 lr_next = (threadIdx.x < num) ? bt.flr[beg + threadIdx.x] : int2{0, 0};
     int cr_next = (threadIdx.x < num && g.crow) ? g.crow[beg + threadIdx.x] : 0;
+    constexpr bool gpre = EARLY && GEO;
+    typedef S S2 __attribute__((ext_vector_type(2)));
+    const S2 *q2 = reinterpret_cast<const S2 *>(g.pairs);
+    S gx_next[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (gpre && threadIdx.x < num) {
+        for (int k = 0; k < 4; k++) {
+            const S2 x = ld_stream(&q2[((long)k * bt.nj) + beg + threadIdx.x]);
+            gx_next[2 * k] = x.x;
+            gx_next[(2 * k) + 1] = x.y;
+        }
+    }
     for (int j = threadIdx.x; j < num; j += BS) {
         const int2 lr = lr_next;
         const int cr = cr_next;
+        S gx[8];
+        for (int i = 0; i < 8; i++) gx[i] = gx_next[i];
         if (j + BS < num) {
             lr_next = bt.flr[beg + j + BS];
             if (g.crow) {
                 cr_next = g.crow[beg + j + BS];
             }
+            if (gpre) {
+                for (int k = 0; k < 4; k++) {
+                    const S2 x = ld_stream(&q2[((long)k * bt.nj) + beg + j + BS]);
+                    gx_next[2 * k] = x.x;
+                    gx_next[(2 * k) + 1] = x.y;
+                }
+            }
         }
         S fv[5], fc[5];
-        face_flux<S, FLUX, VISC, ORDER>(tf, g, ph, P, GP, beg + j, lr, cr, fv, fc, &err);
+        face_flux<S, FLUX, VISC, ORDER, gpre>(tf, g, ph, P, GP, beg + j, lr, cr, fv, fc, &err, gx);
         if (R::MERGE) {
             for (int v = 0; v < 5; v++) lds[(v * mf) + j] = fc[v] - fv[v];
         }

@@ -70,20 +70,35 @@ template <typename S>
 using BlockLaunch = void (*)(hipStream_t, size_t, Topo, BlockTopo, Geom<S>, Phys, const S *,
                              const S *, S *, StageArgs);
 
-// BS rows (and threads) per workgroup: the block layout the face lists were built for
+template <typename S, int FLUX, int VISC, int ORDER, int MODE, int BS, int GEO>
+static void launch_block_geo(hipStream_t s, size_t lds, Topo t, BlockTopo bt, Geom<S> g, Phys ph,
+                             const S *P, const S *GP, S *Pout, StageArgs a)
+{
+    static size_t granted = 64 * 1024;  // dynamic LDS above 64 KiB has to be requested
+    if (lds > granted &&
+        hipFuncSetAttribute((const void *)k_flux_block<S, FLUX, VISC, ORDER, MODE, BS, GEO>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess) {
+        granted = lds;
+    }
+    hipLaunchKernelGGL((k_flux_block<S, FLUX, VISC, ORDER, MODE, BS, GEO>),
+                       dim3((unsigned)((t.ni + BS - 1) / BS)), dim3(BS), lds, s, t, bt, g, ph, P,
+                       GP, Pout, a);
+}
+
+// BS rows (and threads) per workgroup: the block layout the face lists were built for.  FP64
+// with viscous and convective records and rebuilt tangents: the instance that loads the next
+// face's geometry records ahead.
 template <typename S, int FLUX, int VISC, int ORDER, int MODE, int BS>
 static void launch_block(hipStream_t s, size_t lds, Topo t, BlockTopo bt, Geom<S> g, Phys ph,
                          const S *P, const S *GP, S *Pout, StageArgs a)
 {
-    static size_t granted = 64 * 1024;  // dynamic LDS above 64 KiB has to be requested
-    if (lds > granted &&
-        hipFuncSetAttribute((const void *)k_flux_block<S, FLUX, VISC, ORDER, MODE, BS>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess) {
-        granted = lds;
+    if constexpr (sizeof(S) == 8 && VISC && FLUX != FLUX_NONE) {
+        if (g.pairs && g.tangents) {
+            launch_block_geo<S, FLUX, VISC, ORDER, MODE, BS, 1>(s, lds, t, bt, g, ph, P, GP, Pout, a);
+            return;
+        }
     }
-    hipLaunchKernelGGL((k_flux_block<S, FLUX, VISC, ORDER, MODE, BS>),
-                       dim3((unsigned)((t.ni + BS - 1) / BS)), dim3(BS), lds, s, t, bt, g, ph, P,
-                       GP, Pout, a);
+    launch_block_geo<S, FLUX, VISC, ORDER, MODE, BS, 0>(s, lds, t, bt, g, ph, P, GP, Pout, a);
 }
 
 template <typename S, int FLUX, int VISC, int ORDER, int BS>
