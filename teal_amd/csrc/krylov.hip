@@ -446,9 +446,12 @@ __device__ inline unsigned sync_load(const unsigned *p)
 // its own stores (s_waitcnt covers only the wave that issues it), then thread 0 arrives for the
 // workgroup.  `sync` holds, 256 bytes
 // apart: eight arrival counters (workgroup b counts on counter b % 8), the counter of completed
-// groups, the generation and a failure flag.  The last arrival of a counter resets it and
-// counts its group; the last group resets that count and advances the generation, which the
-// waiting workgroups poll.  A wait gives up after spin_limit polls (ARNOLDI_SPIN_LIMIT, about a
+// groups, the generation and a failure flag.  The counters only grow (modulo 2^32): after g
+// barriers since the words were cleared, counter c holds g times its members and the group
+// counter g times the number of groups, so the last arrival of a counter (the one that brings
+// it to (g + 1) times its members) counts its group, and the last group advances the
+// generation, which the waiting workgroups poll; no reset sits on that path.  Every launch
+// that uses the words has the same grid.  A wait gives up after spin_limit polls (ARNOLDI_SPIN_LIMIT, about a
 // second, unless a test lowers it) and raises the failure flag, which ends every other wait at
 // once and makes later launches return at their start, and err_host (host-coherent memory), so
 // that the kernel ends whatever happens; the host then reports the failure and clears the
@@ -468,15 +471,9 @@ __device__ inline void grid_barrier(unsigned *sync, unsigned &gen, int *err_host
         const unsigned c = blockIdx.x & 7;
         const unsigned members = (nb - c + 7) / 8;
         const unsigned groups = nb < 8 ? nb : 8;
-        bool release = false;
-        if (atomicAdd(&sync[c * SYNC_STRIDE], 1u) == members - 1) {
-            atomicExch(&sync[c * SYNC_STRIDE], 0u);
-            wait_memory();
-            release = atomicAdd(&sync[8 * SYNC_STRIDE], 1u) == groups - 1;
-        }
-        if (release) {
-            atomicExch(&sync[8 * SYNC_STRIDE], 0u);
-            wait_memory();
+        const unsigned next = gen + 1u;
+        if (atomicAdd(&sync[c * SYNC_STRIDE], 1u) + 1u == next * members &&
+            atomicAdd(&sync[8 * SYNC_STRIDE], 1u) + 1u == next * groups) {
             atomicAdd(&sync[SYNC_GEN], 1u);
         }
         else {
