@@ -444,18 +444,18 @@ __device__ inline unsigned sync_load(const unsigned *p)
 // Grid-wide barrier of k_arnoldi, whose workgroups are all resident; lane 0 of each group of RT
 // threads has stored its partial with store_part before the call.  Every wave first waits for
 // its own stores (s_waitcnt covers only the wave that issues it), then thread 0 arrives for the
-// workgroup.  `sync` holds, 256 bytes
-// apart: eight arrival counters (workgroup b counts on counter b % 8), the counter of completed
-// groups, the generation and a failure flag.  The counters only grow (modulo 2^32): after g
-// barriers since the words were cleared, counter c holds g times its members and the group
-// counter g times the number of groups, so the last arrival of a counter (the one that brings
-// it to (g + 1) times its members) counts its group, and the last group advances the
-// generation, which the waiting workgroups poll; no reset sits on that path.  Every launch
-// that uses the words has the same grid.  A wait gives up after spin_limit polls (ARNOLDI_SPIN_LIMIT, about a
-// second, unless a test lowers it) and raises the failure flag, which ends every other wait at
-// once and makes later launches return at their start, and err_host (host-coherent memory), so
-// that the kernel ends whatever happens; the host then reports the failure and clears the
-// words.
+// workgroup.  `sync` holds, 256 bytes apart: eight arrival counters (workgroup b counts on
+// counter b % 8), the counter of completed groups, the generation and a failure flag.  The
+// counters only grow (modulo 2^32): after g barriers since the words were cleared, counter c
+// holds g times its members and the group counter g times the number of groups, so the last
+// arrival of a counter (the one that brings it to (g + 1) times its members) counts its group,
+// and the last group advances the generation, which the waiting workgroups poll; no reset
+// sits on that path.  Every launch that uses the words has the same grid.  (1, 2 and 4
+// counters were slower than 8.)  A wait gives up after spin_limit polls (ARNOLDI_SPIN_LIMIT,
+// about a second, unless a test lowers it) and raises the failure flag, which ends every other
+// wait at once and makes later launches return at their start, and err_host (host-coherent
+// memory), so that the kernel ends whatever happens; the host then reports the failure and
+// clears the words.
 constexpr int SYNC_STRIDE = 64;  // unsigned words: 256 bytes
 constexpr int SYNC_GEN = 9 * SYNC_STRIDE;
 constexpr int SYNC_FAILED = 10 * SYNC_STRIDE;
