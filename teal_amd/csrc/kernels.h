@@ -402,9 +402,10 @@ __global__ __launch_bounds__(GB) void k_grad_limit(Topo t, Geom<S> g, const S *_
 }
 
 // ---------------------------------------------------------------- face fluxes
-// Viscous (fv) and convective (fc) contribution F*A of face f.
-// GPRE: FP64 records with rebuilt tangents (g.tangents), their 4 pairs loaded by the caller
-// into `pre` (one face ahead).
+// Viscous (fv) and convective (fc) contribution F*A of face f, whose cells are lr (left,
+// right or ~ghost) and, in the centre form, whose right cell's centre is row `crow` (g.crow[f],
+// loaded by the caller).  GPRE: FP64 records with rebuilt tangents (g.tangents), their 4 pairs
+// loaded by the caller into `pre` (one face ahead).
 template <typename S, int FLUX, int VISC, int ORDER, bool GPRE = false>
 __device__ inline void face_flux(const Topo &t, const Geom<S> &g, const Phys &ph,
                                  const S *__restrict__ P, const S *__restrict__ GP, int f,
