@@ -362,3 +362,44 @@ def test_hex_geometry_bit_exact(h, periodic):
     for c in range(ni):
         for e in range(coff[c], coff[c + 1]):
             np.testing.assert_array_equal(cof[e], (x[cidx[e]] - x[c]) / 2)
+
+
+def _tangents_from_normal(n):
+    """The reference's tangent construction (src/mesh/generate.c:420-435) from normals n [k][3],
+    and which branch each face takes."""
+    nx, ny, nz = n[:, 0], n[:, 1], n[:, 2]
+    nqz, nqy = np.hypot(nx, ny), np.hypot(nx, nz)
+    first = nqz > nqy
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t1 = np.where(first[:, None], np.stack([-ny / nqz, nx / nqz, 0 * nx], 1),
+                      np.stack([-nx * ny / nqy, nqy, -ny * nz / nqy], 1))
+        t2 = np.where(first[:, None], np.stack([-nx * nz / nqz, -ny * nz / nqz, nqz], 1),
+                      np.stack([-nz / nqy, 0 * nx, nx / nqy], 1))
+    return t1, t2, first
+
+
+@pytest.mark.parametrize("kind", ["box", "periodic", "tets", "mixed"])
+def test_face_tangents_follow_the_construction(kind):
+    """The stored face bases follow generate.c:420-451: tangents built from the unit normal, then
+    the normal and the first tangent negated where the normal pointed into the left cell.  So
+    the construction applied to the stored normal gives the stored tangents, or (second branch
+    of a flipped face) both negated.  The FP64 device records rebuild the tangents this way."""
+    if kind == "tets":
+        a = tet_arrays()
+    elif kind == "mixed":
+        from mixed_mesh import mixed_box, oracle_mixed_mesh
+
+        coord, cells, ni, names, off = mixed_box((4, 3, 2), (1.0, 0.5, 0.25), {(1, 1), (2, 0)})
+        m = oracle_mixed_mesh(coord, cells, ni, names, off)
+        oracle().oracle_mesh_generate(m)
+        a = _abi.mesh_arrays(m)
+    else:
+        a = hex_mesh(*BOXES[0 if kind == "box" else 1])
+    b = np.asarray(a["basis"]).reshape(-1, 9)
+    t1, t2, first = _tangents_from_normal(b[:, 0:3])
+    same = np.all(t1 == b[:, 3:6], 1) & np.all(t2 == b[:, 6:9], 1)
+    negated = np.all(t1 == -b[:, 3:6], 1) & np.all(t2 == -b[:, 6:9], 1)
+    assert np.all(same | (negated & ~first))
+    assert np.any(negated & ~same)  # some faces are flipped second-branch faces
+    if kind == "mixed":
+        oracle().oracle_mesh_destroy(m)
