@@ -196,6 +196,9 @@ def accuracy(g64, gmx, P0, steps, volume):
     return {"steps": steps, "time_fp64": t64, "time_mixed": tmx,
             "normalisation": "volume-weighted RMS of the FP64 field per conserved variable; "
                              "momentum components by the RMS momentum magnitude",
+            "mixed_arithmetic": "FP32 face fluxes, gradients, limiters and geometry; each FP32 "
+                                "face contribution summed into its cell in FP64 (exact, in "
+                                "place of compensated FP32 sums); FP64 RK update and U",
             "l2": dict(zip(names, l2.tolist())),
             "l2_relative": dict(zip(names, rel.tolist())),
             "linf": dict(zip(names, linf.tolist())),
@@ -391,7 +394,7 @@ def tet_workload(args, n=96, seed=2024):
         "value": best["cell_updates_per_s"], "unit": "cell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": best["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 flux/gradient/geometry + f64 update (mixed)", "data": "synthetic",
+        "dtype": "f32 flux/gradient/geometry + f64 accumulation and update (mixed)", "data": "synthetic",
         "config": {"workload": f"Euler Mach-2 bump, {n}^3 hexes x 6 jittered tets ({ni} cells), "
                                "Roe, LSQ + Barth-Jespersen, LSERK 2/3, CFL 0.5",
                    "cells": ni, "faces": nf},
@@ -743,7 +746,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 flux/gradient/geometry + f64 update (mixed)",
+        "dtype": "f32 flux/gradient/geometry + f64 accumulation and update (mixed)",
         "data": "synthetic",
         "config": {
             "workload": f"Navier-Stokes Taylor-Green vortex, periodic {n}^3 hex mesh on [0,2pi]^3 "
