@@ -79,6 +79,15 @@ def test_rebuilt_tangents_match_stored_basis(name, case, kind, monkeypatch):
     assert np.abs(R1).max() > 0
     np.testing.assert_array_equal(R1, R2)
     np.testing.assert_array_equal(P1, P2)
+    # RK stages (the stage kernels' update and last-stage modes) on both forms
+    rebuilt.upload(P)
+    stored.upload(P)
+    t1 = t2 = 0.0
+    for _ in range(2):
+        _, t1, res1 = rebuilt.lserk(t1, 0.5)
+        _, t2, res2 = stored.lserk(t2, 0.5)
+        assert t1 == t2 and tuple(res1) == tuple(res2)
+    np.testing.assert_array_equal(rebuilt.download(), stored.download())
 
 
 LSERK_CASES = RESIDUAL_CASES[:3] + RESIDUAL_CASES[12:15] + LARGE_CASES
