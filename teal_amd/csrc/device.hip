@@ -703,7 +703,10 @@ int DevData<S>::build_blocks(BlockSet<S> &bl, int rows, const std::vector<int> &
     // FP64: the centre form (48 fewer bytes per face) where it reproduces the stored geometry.
     // FP32 keeps the offsets rounded from FP64 (a difference of FP32 centres loses digits) but
     // derives the correction from them and the tangents from the normal (40 fewer bytes).
-    bl.centre_form = sizeof(S) == 8 && centres_reproduce(flr, crow, fcen, ccen, offl, offr, corr);
+    // (TEAL_GPU_STORED_OFFSETS=1 at creation keeps the stored offsets and corrections, the form
+    // of meshes whose centres do not reproduce them)
+    bl.centre_form = sizeof(S) == 8 && !getenv("TEAL_GPU_STORED_OFFSETS") &&
+                     centres_reproduce(flr, crow, fcen, ccen, offl, offr, corr);
     const bool centre_form = bl.centre_form;
     const bool fp64 = sizeof(S) == 8;
     const int brows = fp64 ? 9 : 3;

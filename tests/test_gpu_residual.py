@@ -90,6 +90,36 @@ def test_rebuilt_tangents_match_stored_basis(name, case, kind, monkeypatch):
     np.testing.assert_array_equal(rebuilt.download(), stored.download())
 
 
+@pytest.mark.parametrize("name,case,kind", RESIDUAL_CASES, ids=[c[0] for c in RESIDUAL_CASES])
+def test_stored_offsets_form_matches_oracle(name, case, kind, monkeypatch):
+    """FP64 face records with the stored offsets and corrections (the form of meshes whose cell
+    and face centres do not reproduce them; TEAL_GPU_STORED_OFFSETS=1 forces it here): the
+    residual equals the oracle's, and two RK steps equal those of the centre form, bit for bit."""
+    print(f"case {name} (stored offsets)", flush=True)
+    monkeypatch.delenv("TEAL_GPU_SPLIT", raising=False)
+    monkeypatch.delenv("TEAL_GPU_STORED_OFFSETS", raising=False)
+    _, _, centre, _, _ = setup_pair(case, kind)
+    monkeypatch.setenv("TEAL_GPU_STORED_OFFSETS", "1")
+    om, oe, stored, arrays, P = setup_pair(case, kind)
+    assert centre.info()["centre_form"] == 1
+    assert stored.info()["centre_form"] == 0 and stored.info()["tangents"] == 0
+    Ro = np.zeros((arrays["num_inner"], 5))
+    oracle().oracle_derivative(oe, dp(P.copy()), dp(Ro))
+    Rg, _ = stored.derivative(P.copy())
+    if case.flux == "godunov":  # device pow (see test_derivative_bitwise)
+        np.testing.assert_allclose(Rg, Ro, rtol=1e-12, atol=1e-12 * np.abs(Ro).max())
+    else:
+        np.testing.assert_array_equal(Rg, Ro)
+    centre.upload(P)
+    stored.upload(P)
+    t1 = t2 = 0.0
+    for _ in range(2):
+        _, t1, res1 = centre.lserk(t1, 0.5)
+        _, t2, res2 = stored.lserk(t2, 0.5)
+        assert t1 == t2 and tuple(res1) == tuple(res2)
+    np.testing.assert_array_equal(centre.download(), stored.download())
+
+
 LSERK_CASES = RESIDUAL_CASES[:3] + RESIDUAL_CASES[12:15] + LARGE_CASES
 
 
